@@ -1,0 +1,181 @@
+"""Oracle for the tiled Cholesky path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this package.  It wraps ``libcham_oracle.so`` (the C
+restatement of Chameleon's CPU path, see cham_oracle.c) and restates
+Chameleon's numerical acceptance checks in numpy:
+
+* :func:`check_factorization` -- testing/testing_zcheck_facto.c:198-300,
+  ``||A0 - L L^T||_1 / (||A0||_1 N eps) <= 60`` with the symmetric one-norm
+  of the ``uplo`` triangle (dlansy).
+* :func:`check_solve` -- testing/testing_zcheck_facto.c:392-450,
+  ``||B - A X||_1 / (||A||_1 ||X||_1 max(M,N) eps) <= 60``.
+* ``EPS = 2**-53`` -- control/lamch.c:49-53 (LAPACK dlamch('e')).
+"""
+import ctypes
+import glob
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcham_oracle.so")
+
+UPPER, LOWER, UPPERLOWER = 121, 122, 123
+EPS = 2.0 ** -53
+THRESHOLD = 60.0
+
+_lib = None
+
+
+def build():
+    """Compile the C restatement (gcc, OpenMP)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "libcham_oracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        u64, i32, dbl, ptr = ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        L.orc_rnd64_jump.restype = u64
+        L.orc_rnd64_jump.argtypes = [u64, u64]
+        L.orc_dlaran_at.restype = dbl
+        L.orc_dlaran_at.argtypes = [u64, u64]
+        L.orc_dplgsy.argtypes = [dbl, i32, i32, ptr, i32, i32, u64]
+        L.orc_dplrnt.argtypes = [i32, i32, ptr, i32, i32, u64]
+        L.orc_dplgsy_tile.argtypes = [dbl, i32, i32, ptr, i32, i32, i32, i32, u64]
+        L.orc_dplrnt_tile.argtypes = [i32, i32, ptr, i32, i32, i32, i32, u64]
+        L.orc_dpotrf_tile.argtypes = [i32, i32, ptr, i32]
+        L.orc_dtrsm_tile.argtypes = [i32, i32, i32, i32, i32, i32, dbl, ptr, i32, ptr, i32]
+        L.orc_dsyrk_tile.argtypes = [i32, i32, i32, i32, dbl, ptr, i32, dbl, ptr, i32]
+        L.orc_dgemm_tile.argtypes = [i32, i32, i32, i32, i32, dbl, ptr, i32, ptr, i32, dbl, ptr, i32]
+        L.orc_dpotrf.argtypes = [i32, i32, ptr, i32, i32, i32]
+        L.orc_dpotrs.argtypes = [i32, i32, i32, ptr, i32, ptr, i32, i32, i32]
+        L.orc_dposv.argtypes = [i32, i32, i32, ptr, i32, ptr, i32, i32, i32]
+        L.orc_set_blas.argtypes = [ptr, ptr, ptr, ptr, ptr]
+        L.orc_wtime.restype = dbl
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _fortran(shape):
+    return np.zeros(shape, dtype=np.float64, order="F")
+
+
+def rnd64_jump(n, seed):
+    return lib().orc_rnd64_jump(n, seed)
+
+
+def dplgsy(bump, uplo, n, seed, lda=None, nb=None, out=None):
+    """Column-major symmetric random matrix of CHAMELEON_dplgsy."""
+    lda = lda or max(1, n)
+    A = out if out is not None else _fortran((lda, n))
+    lib().orc_dplgsy(float(bump), uplo, n, _p(A), lda, nb or max(1, n), seed)
+    return A
+
+
+def dplrnt(m, n, seed, lda=None, nb=None, out=None):
+    """Column-major uniform random matrix of CHAMELEON_dplrnt."""
+    lda = lda or max(1, m)
+    A = out if out is not None else _fortran((lda, n))
+    lib().orc_dplrnt(m, n, _p(A), lda, nb or max(1, m), seed)
+    return A
+
+
+def dpotrf(uplo, A, n, nb, nthreads=0):
+    """In-place tiled Cholesky of the leading n x n block of Fortran-ordered A."""
+    assert A.flags.f_contiguous and A.dtype == np.float64
+    return lib().orc_dpotrf(uplo, n, _p(A), A.shape[0], nb, nthreads)
+
+
+def dpotrs(uplo, A, n, B, nrhs, nb, nthreads=0):
+    assert A.flags.f_contiguous and B.flags.f_contiguous
+    return lib().orc_dpotrs(uplo, n, nrhs, _p(A), A.shape[0], _p(B), B.shape[0], nb, nthreads)
+
+
+def dposv(uplo, A, n, B, nrhs, nb, nthreads=0):
+    assert A.flags.f_contiguous and B.flags.f_contiguous
+    return lib().orc_dposv(uplo, n, nrhs, _p(A), A.shape[0], _p(B), B.shape[0], nb, nthreads)
+
+
+def find_openblas():
+    """Path of the LP64 OpenBLAS shipped with scipy (Fortran interface with a
+    ``scipy_`` prefix), or None."""
+    try:
+        import scipy
+    except ImportError:
+        return None
+    root = os.path.dirname(os.path.dirname(scipy.__file__))
+    hits = sorted(glob.glob(os.path.join(root, "scipy.libs", "libscipy_openblas-*.so")))
+    return hits[0] if hits else None
+
+
+_blas_handle = None
+
+
+def use_blas(enable=True):
+    """Route the oracle's tile kernels through host OpenBLAS (single-threaded
+    inside each task, as coreblas assumes) or back to the textbook loops."""
+    global _blas_handle
+    L = lib()
+    if not enable:
+        L.orc_set_blas(None, None, None, None, None)
+        return None
+    path = find_openblas()
+    if path is None:
+        return None
+    _blas_handle = ctypes.CDLL(path)
+    h = _blas_handle
+    L.orc_set_blas(ctypes.cast(h.scipy_dpotrf_, ctypes.c_void_p), ctypes.cast(h.scipy_dtrsm_, ctypes.c_void_p),
+                   ctypes.cast(h.scipy_dsyrk_, ctypes.c_void_p), ctypes.cast(h.scipy_dgemm_, ctypes.c_void_p),
+                   ctypes.cast(h.scipy_openblas_set_num_threads, ctypes.c_void_p))
+    return path
+
+
+# ------------------------------------------------------------------ checks
+def dlansy_one(A, uplo):
+    """One-norm of the symmetric matrix whose `uplo` triangle is stored in A."""
+    T = np.tril(A) if uplo == LOWER else np.triu(A)
+    S = T + T.T - np.diag(np.diag(A))
+    return np.abs(S).sum(axis=0).max() if S.size else 0.0
+
+
+def check_factorization(A0, F, uplo):
+    """Chameleon's check_dxxtrf for a symmetric matrix: returns
+    (ok, ||A0 - L L^T||_1 / (||A0||_1 N eps)).  A0 is the full symmetric
+    input, F holds the factor in its `uplo` triangle."""
+    n = A0.shape[0]
+    if uplo == LOWER:
+        L = np.tril(F)
+        R = A0 - L @ L.T
+    else:
+        U = np.triu(F)
+        R = A0 - U.T @ U
+    anorm = dlansy_one(A0, uplo)
+    rnorm = dlansy_one(R, uplo)
+    res = rnorm / (anorm * n * EPS)
+    return (np.isfinite(res) and res <= THRESHOLD), res
+
+
+def check_solve(A0, X, B):
+    """Chameleon's check_dsolve (ChamSymmetric, NoTrans): returns
+    (ok, ||B - A X||_1 / (||A||_1 ||X||_1 N eps))."""
+    n = A0.shape[0]
+    anorm = np.abs(A0).sum(axis=0).max()
+    xnorm = np.abs(X).sum(axis=0).max()
+    rnorm = np.abs(B - A0 @ X).sum(axis=0).max()
+    res = rnorm / (anorm * xnorm * n * EPS)
+    return (np.isfinite(xnorm) and np.isfinite(res) and res <= THRESHOLD), res
+
+
+def rel_frobenius(X, Y):
+    d = np.linalg.norm(Y)
+    return np.linalg.norm(X - Y) / (d if d > 0 else 1.0)
