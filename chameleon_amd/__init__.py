@@ -1,0 +1,240 @@
+"""Python binding of libchameleon_amd.so, the MI355X-native Chameleon
+Cholesky library.
+
+The shared library exports the Chameleon C API (include/chameleon.h); this
+module declares those entry points with ctypes and adds a thin helper layer
+for numpy arrays:
+
+    import chameleon_amd as ch
+    ch.init()
+    A = ch.Desc.from_array(a, nb=384)          # CM descriptor over a Fortran array
+    ch.check(ch.lib.CHAMELEON_dposv_Tile(ch.LOWER, A.ptr, B.ptr))
+
+There is no fallback: if the library is not built, or no GPU is visible,
+every call raises.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libchameleon_amd.so")
+INCLUDE = os.path.join(ROOT, "include")
+
+# enum values (include/chameleon/constants.h)
+UPPER, LOWER, UPPERLOWER = 121, 122, 123
+NOTRANS, TRANS = 111, 112
+ONE_NORM, FROBENIUS_NORM, INF_NORM, MAX_NORM = 171, 174, 175, 177
+REAL_DOUBLE = (1 << 2) | 3
+REAL_FLOAT = (1 << 2) | 2
+MAT_ALLOC_GLOBAL = None
+MAT_ALLOC_TILE = ctypes.c_void_p(-1 & ((1 << 64) - 1))
+TILE_SIZE, LOOKAHEAD = 2, 8
+SUCCESS = 0
+ERR_NOT_INITIALIZED = -101
+ERR_ILLEGAL_VALUE = -104
+COL_MAJOR = 102
+
+
+class ChameleonError(RuntimeError):
+    pass
+
+
+def build_library():
+    """Compile libchameleon_amd.so (see build.py)."""
+    import importlib
+    return importlib.import_module(__name__ + ".build").build()
+
+
+_lib = None
+
+
+def _declare(L):
+    i32, dbl, vp, u64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_ulonglong
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "__chameleon_init": (i32, [i32, i32]),
+        "__chameleon_finalize": (i32, []),
+        "CHAMELEON_Initialized": (i32, []),
+        "CHAMELEON_Set": (i32, [i32, i32]),
+        "CHAMELEON_Get": (i32, [i32, ctypes.POINTER(i32)]),
+        "CHAMELEON_Desc_Create": (i32, [pp, vp, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32]),
+        "CHAMELEON_Desc_Destroy": (i32, [pp]),
+        "CHAMELEON_Desc_Flush": (i32, [vp, vp]),
+        "CHAMELEON_Desc_Acquire": (i32, [vp]),
+        "CHAMELEON_Desc_Release": (i32, [vp]),
+        "CHAMELEON_Lap2Desc": (i32, [i32, vp, i32, vp]),
+        "CHAMELEON_Desc2Lap": (i32, [i32, vp, vp, i32]),
+        "CHAMELEON_Sequence_Create": (i32, [pp]),
+        "CHAMELEON_Sequence_Destroy": (i32, [vp]),
+        "CHAMELEON_Sequence_Wait": (i32, [vp]),
+        "CHAMELEON_dposv": (i32, [i32, i32, i32, vp, i32, vp, i32]),
+        "CHAMELEON_dpotrf": (i32, [i32, i32, vp, i32]),
+        "CHAMELEON_dpotrs": (i32, [i32, i32, i32, vp, i32, vp, i32]),
+        "CHAMELEON_dplgsy": (i32, [dbl, i32, i32, vp, i32, u64]),
+        "CHAMELEON_dplrnt": (i32, [i32, i32, vp, i32, u64]),
+        "CHAMELEON_dlange": (dbl, [i32, i32, i32, vp, i32]),
+        "CHAMELEON_dlansy": (dbl, [i32, i32, i32, vp, i32]),
+        "CHAMELEON_dposv_Tile": (i32, [i32, vp, vp]),
+        "CHAMELEON_dpotrf_Tile": (i32, [i32, vp]),
+        "CHAMELEON_dpotrs_Tile": (i32, [i32, vp, vp]),
+        "CHAMELEON_dplgsy_Tile": (i32, [dbl, i32, vp, u64]),
+        "CHAMELEON_dplrnt_Tile": (i32, [vp, u64]),
+        "CHAMELEON_dlange_Tile": (dbl, [i32, vp]),
+        "CHAMELEON_dlansy_Tile": (dbl, [i32, i32, vp]),
+        "CHAMELEON_dlacpy_Tile": (i32, [i32, vp, vp]),
+        "CHAMELEON_dposv_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
+        "CHAMELEON_dpotrf_Tile_Async": (i32, [i32, vp, vp, vp]),
+        "CHAMELEON_dpotrs_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
+        "CHAMELEON_dplgsy_Tile_Async": (i32, [dbl, i32, vp, u64, vp, vp]),
+        "CHAMELEON_dplrnt_Tile_Async": (i32, [vp, u64, vp, vp]),
+        "CHAMELEON_lapacke_dposv": (i32, [i32, ctypes.c_char, i32, i32, vp, i32, vp, i32]),
+        "CHAMELEON_lapacke_dpotrf": (i32, [i32, ctypes.c_char, i32, vp, i32]),
+        "CHAMELEON_lapacke_dpotrs": (i32, [i32, ctypes.c_char, i32, i32, vp, i32, vp, i32]),
+        "chameleon_lapack_dposv_": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "chameleon_lapack_dpotrf_": (None, [vp, vp, vp, vp, vp]),
+        "chameleon_lapack_dpotrs_": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def load():
+    """Load libchameleon_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChameleonError(f"{LIB_PATH} is missing: run chameleon_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+class _LibProxy:
+    def __getattr__(self, name):
+        return getattr(load(), name)
+
+
+lib = _LibProxy()
+
+
+def check(rc, what="call"):
+    if rc != SUCCESS:
+        raise ChameleonError(f"{what} returned {rc}")
+    return rc
+
+
+def init(ncpus=1, ngpus=1):
+    L = load()
+    if L.CHAMELEON_Initialized():
+        return
+    check(L.__chameleon_init(ncpus, ngpus), "CHAMELEON_Init")
+
+
+def finalize():
+    L = load()
+    if L.CHAMELEON_Initialized():
+        check(L.__chameleon_finalize(), "CHAMELEON_Finalize")
+
+
+def set_param(param, value):
+    check(load().CHAMELEON_Set(param, value), "CHAMELEON_Set")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Desc:
+    """Owning wrapper of a CHAM_desc_t*.  `array` keeps the user's host
+    matrix alive for CM descriptors."""
+
+    def __init__(self, ptr, array=None):
+        self.ptr = ptr
+        self.array = array
+
+    @classmethod
+    def from_array(cls, a, nb, m=None, n=None):
+        """CM descriptor over the Fortran-ordered float64 array `a`
+        (lm = a.shape[0] is the leading dimension)."""
+        assert a.dtype == np.float64 and a.flags.f_contiguous
+        lm, ln = a.shape
+        m = lm if m is None else m
+        n = ln if n is None else n
+        p = ctypes.c_void_p()
+        check(load().CHAMELEON_Desc_Create(ctypes.byref(p), _ptr(a), REAL_DOUBLE, nb, nb, nb * nb, lm, ln, 0, 0, m, n,
+                                           1, 1), "CHAMELEON_Desc_Create")
+        return cls(p, a)
+
+    @classmethod
+    def alloc(cls, m, n, nb, dtype=REAL_DOUBLE):
+        """Library-allocated (CCRB) descriptor."""
+        p = ctypes.c_void_p()
+        check(load().CHAMELEON_Desc_Create(ctypes.byref(p), MAT_ALLOC_TILE, dtype, nb, nb, nb * nb, m, n, 0, 0, m, n,
+                                           1, 1), "CHAMELEON_Desc_Create")
+        return cls(p, None)
+
+    def to_array(self, m, n, uplo=UPPERLOWER, out=None):
+        a = out if out is not None else np.zeros((m, n), order="F")
+        check(load().CHAMELEON_Desc2Lap(uplo, self.ptr, _ptr(a), a.shape[0]), "CHAMELEON_Desc2Lap")
+        return a
+
+    def destroy(self):
+        if self.ptr:
+            load().CHAMELEON_Desc_Destroy(ctypes.byref(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+class Sequence:
+    def __init__(self):
+        self.ptr = ctypes.c_void_p()
+        check(load().CHAMELEON_Sequence_Create(ctypes.byref(self.ptr)), "CHAMELEON_Sequence_Create")
+        # RUNTIME_request_t { int status, flush, priority; void *parent }
+        self.request = (ctypes.c_char * 24)()
+        ctypes.memset(self.request, 0, 24)
+        ctypes.c_int.from_buffer(self.request, 4).value = 1
+
+    @property
+    def req(self):
+        return ctypes.cast(self.request, ctypes.c_void_p)
+
+    def wait(self):
+        return load().CHAMELEON_Sequence_Wait(self.ptr)
+
+    def destroy(self):
+        if self.ptr:
+            load().CHAMELEON_Sequence_Destroy(self.ptr)
+            self.ptr = None
+
+
+def header_functions():
+    """Names of every function declared in include/ (the exported C ABI)."""
+    names = set()
+    for dirpath, _, files in os.walk(INCLUDE):
+        for f in files:
+            if not f.endswith(".h"):
+                continue
+            text = open(os.path.join(dirpath, f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            text = re.sub(r"static inline[^{]*\{[^}]*\}", "", text, flags=re.S)
+            for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+                name = m.group(1)
+                if name in ("if", "while", "for", "return", "sizeof", "defined", "typedef"):
+                    continue
+                line = text[m.start():text.find("\n", m.start())]
+                if line.lstrip().startswith(("#", "typedef")):
+                    continue
+                names.add(name)
+    return names

@@ -1,0 +1,328 @@
+/*
+ * cholesky.cpp -- the single-GPU scheduler of the tiled Cholesky
+ * factor-and-solve path and its tile-layout entry points.
+ *
+ * Factorisation (Chameleon: chameleon_pzpotrf, compute/pzpotrf.c:84-193).
+ * For each panel k the reference submits POTRF(k,k), TRSM(m,k) for m > k,
+ * SYRK(n,n) and GEMM(m,n) for k < n <= m, and a runtime (StarPU, OpenMP)
+ * orders them by data dependencies.  Here the same right-looking order is
+ * issued as three kinds of launch per step on two HIP streams:
+ *
+ *   critical stream  : update column k+1 with panel k   (trailing_update, colmode)
+ *                      POTRF of tile (k+1,k+1)           (potrf_diag, 1 workgroup)
+ *                      TRSM of tiles (m,k+1), m > k+1     (trsm_right, 1 wave / 16 rows)
+ *   update stream    : update columns k+2..NT-1 with panel k  (trailing_update, triangle)
+ *
+ * with events so that panel k+1 is factored while the bulk of step k's
+ * trailing update still runs (lookahead of one panel, the role of the task
+ * priorities 2NT-2k[-m][-n] in pzpotrf.c:90-123).  With
+ * CHAMELEON_LOOKAHEAD = 0 everything is issued on one stream.  The Upper
+ * variant runs on the transposed device mirror (U = L^T).
+ *
+ * Solves (chameleon_pztrsm, compute/pztrsm.c:134-201): the inverse of every
+ * diagonal tile is formed once (trsm_right on identity tiles), then each
+ * step k multiplies the solved block by it and updates the blocks below
+ * (forward) or above (backward) in one strided batched GEMM launch.
+ */
+#include "internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+using namespace chx;
+
+namespace chx {
+
+static hipEvent_t event(size_t i)
+{
+    Context *c = ctx();
+    while (c->events.size() <= i) {
+        hipEvent_t e;
+        CHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->events.push_back(e);
+    }
+    return c->events[i];
+}
+
+struct PotrfPlan {
+    double *A;
+    int NT, ldt, nb, nvalid_last;   /* nb: descriptor tile size (valid columns of a full tile) */
+    int *d_info;
+};
+
+static void potrf_run(const PotrfPlan &p)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main, su = c->s_update;
+    const int NT = p.NT, ldt = p.ldt;
+    const size_t ts = (size_t)ldt * ldt;
+    const int nblk = ldt / 16;
+    double *dinv = (double *)workspace((size_t)NT * nblk * 256 * sizeof(double));
+    auto tile = [&](int m, int n) { return p.A + ((size_t)n * NT + m) * ts; };
+    auto panel = [&](int k) {
+        const int nvalid = (k == NT - 1) ? p.nvalid_last : p.nb;
+        double *dk = dinv + (size_t)k * nblk * 256;
+        k::potrf_diag(tile(k, k), ldt, dk, p.nb * k, nvalid, p.d_info, sm);
+        if (k + 1 < NT) k::trsm_right_lower_trans(tile(k + 1, k), NT - k - 1, ts, ldt, tile(k, k), dk, p.d_info, sm);
+    };
+    if (c->lookahead <= 0 || NT <= 2) {
+        for (int k = 0; k < NT; ++k) {
+            panel(k);
+            k::trailing_update(p.A, NT, ldt, k, k + 1, NT, NT, p.d_info, sm);
+        }
+        return;
+    }
+    /* events: P(k) = 2k (panel k done), R(k) = 2k+1 (rest update of step k done) */
+    /* the update stream must not start before earlier work on the main stream */
+    CHX_HIP(hipEventRecord(event(0), sm));
+    CHX_HIP(hipStreamWaitEvent(su, event(0), 0));
+    panel(0);
+    CHX_HIP(hipEventRecord(event(2 * 0), sm));
+    for (int k = 0; k < NT - 1; ++k) {
+        /* update stream: columns k+2.. with panel k */
+        CHX_HIP(hipStreamWaitEvent(su, event(2 * k), 0));
+        if (k + 2 < NT) k::trailing_update(p.A, NT, ldt, k, k + 2, NT, NT, p.d_info, su);
+        CHX_HIP(hipEventRecord(event(2 * k + 1), su));
+        /* critical stream: column k+1, then panel k+1 */
+        if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
+        k::trailing_update(p.A, NT, ldt, k, k + 1, k + 2, NT, p.d_info, sm);
+        panel(k + 1);
+        CHX_HIP(hipEventRecord(event(2 * (k + 1)), sm));
+    }
+    /* join: the last rest update ends before anything else on the main stream */
+    CHX_HIP(hipStreamWaitEvent(sm, event(2 * (NT - 2) + 1), 0));
+}
+
+/* Forward (trans = 0: L Y = B) or backward (trans = 1: L^T X = Y) solve of
+ * every tile column of B with the Lower factor in A; Tinv holds, for every
+ * diagonal tile k, Linv_k^T (upper triangular). */
+static void solve_run(double *A, int NT, int ldt, const double *Tinv, double *B, int Bnt, int wdt, int trans,
+                      int *d_info)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    const size_t ts = (size_t)ldt * ldt;
+    const size_t bts = (size_t)ldt * wdt;
+    /* W (one B tile) lives right after the Tinv tiles in the workspace */
+    double *Wb = const_cast<double *>(Tinv) + (size_t)NT * ts;
+    for (int bn = 0; bn < Bnt; ++bn) {
+        double *Bc = B + (size_t)bn * NT * bts;   /* tile column bn of B: tiles (0..NT-1, bn) */
+        for (int kk = 0; kk < NT; ++kk) {
+            const int k = trans ? NT - 1 - kk : kk;
+            const double *Tk = Tinv + (size_t)k * ts;
+            double *Bk = Bc + (size_t)k * bts;
+            /* W = op(Linv_k) B_k: forward uses Linv = Tk^T, backward Linv^T = Tk */
+            k::gemm_batched(trans ? 0 : 1, ldt, wdt, ldt, 1.0, Tk, ldt, 0, Bk, ldt, 0, 0.0, Wb, ldt, 0, 1, nullptr,
+                            d_info, sm);
+            if (!trans) {
+                /* B_m -= A(m,k) W for m > k, and B_k <- W */
+                const int cnt = NT - k - 1;
+                k::gemm_batched(0, ldt, wdt, ldt, -1.0, A + ((size_t)k * NT + k + 1) * ts, ldt, (long long)ts, Wb,
+                                ldt, 0, 1.0, Bc + (size_t)(k + 1) * bts, ldt, (long long)bts, cnt, Bk, d_info, sm);
+            } else {
+                /* B_m -= A(k,m)^T W for m < k, and B_k <- W */
+                k::gemm_batched(1, ldt, wdt, ldt, -1.0, A + (size_t)k * ts, ldt, (long long)NT * ts, Wb, ldt, 0,
+                                1.0, Bc, ldt, (long long)bts, k, Bk, d_info, sm);
+            }
+        }
+    }
+}
+
+/* Tinv_k = Linv_k^T for every diagonal tile, into the workspace.
+ * Returns the workspace base (Tinv tiles, then one B-tile-sized W). */
+static double *form_inverses(double *A, int NT, int ldt, int wdt, int *d_info)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    const size_t ts = (size_t)ldt * ldt;
+    const int nblk = ldt / 16;
+    const size_t dinv_elems = (size_t)NT * nblk * 256;
+    const size_t need = ((size_t)NT * ts + (size_t)ldt * wdt + dinv_elems) * sizeof(double);
+    double *ws = (double *)workspace(need);
+    double *Tinv = ws;
+    double *dinv = ws + (size_t)NT * ts + (size_t)ldt * wdt;
+    k::dinv_all(A, NT, (size_t)(NT + 1) * ts, ldt, dinv, sm);
+    k::set_identity(Tinv, NT, ldt, sm);
+    k::trsm_right_lower_trans_batched(Tinv, NT, (long long)ts, ldt, A, (long long)(NT + 1) * ts, dinv,
+                                      (long long)nblk * 256, d_info, sm);
+    return Tinv;
+}
+
+static int check_posv_descs(const char *fn, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence,
+                            RUNTIME_request_t *request)
+{
+    if (!ctx()) {
+        error(fn, "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    if (!sequence) {
+        error(fn, "NULL sequence");
+        return CHAMELEON_ERR_UNALLOCATED;
+    }
+    if (!request) {
+        error(fn, "NULL request");
+        return CHAMELEON_ERR_UNALLOCATED;
+    }
+    if (sequence->status == CHAMELEON_SUCCESS) request->status = CHAMELEON_SUCCESS;
+    else return request_fail(sequence, request, CHAMELEON_ERR_SEQUENCE_FLUSHED);
+    if (desc_check(A) != CHAMELEON_SUCCESS) {
+        error(fn, "invalid first descriptor");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
+    if (B && desc_check(B) != CHAMELEON_SUCCESS) {
+        error(fn, "invalid second descriptor");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
+    if (A->nb != A->mb || (B && B->nb != B->mb)) {
+        error(fn, "only square tiles supported");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
+    if (cham_get_flttype(A->dtyp) != ChamRealDouble || (B && cham_get_flttype(B->dtyp) != ChamRealDouble)) {
+        error(fn, "descriptors must hold real double precision values");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
+    if (A->m != A->n) {
+        error(fn, "matrix A must be square");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
+    if (B && (B->m != A->n || B->mb != A->mb)) {
+        error(fn, "B must have as many rows (and the same tiling) as A");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
+    return CHAMELEON_SUCCESS;
+}
+
+int potrf_async(cham_uplo_t uplo, CHAM_desc_t *A, RUNTIME_sequence_t *sequence)
+{
+    Context *c = ctx();
+    if (A->m == 0) return CHAMELEON_SUCCESS;
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main);
+    Mirror *M = mirror_of(A);
+    SeqState *st = seq_state(sequence);
+    PotrfPlan p;
+    p.A = (double *)M->d;
+    p.NT = M->mt;
+    p.ldt = M->ldt;
+    p.nb = A->nb;
+    p.nvalid_last = A->m - (M->mt - 1) * A->mb;
+    p.d_info = st->d_info;
+    potrf_run(p);
+    st->pending_info = true;
+    mirror_mark_written(A);
+    A->sync = 1;
+    return CHAMELEON_SUCCESS;
+}
+
+int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence)
+{
+    Context *c = ctx();
+    if (A->m == 0 || B->n == 0) return CHAMELEON_SUCCESS;
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main);
+    mirror_acquire(B, MIRROR_NORMAL, false, c->s_main);
+    Mirror *MA = mirror_of(A), *MB = mirror_of(B);
+    SeqState *st = seq_state(sequence);
+    if (MB->ldt != MA->ldt) {
+        error("CHAMELEON_dpotrs", "B tiling does not match A");
+        return CHAMELEON_ERR_ILLEGAL_VALUE;
+    }
+    double *Tinv = form_inverses((double *)MA->d, MA->mt, MA->ldt, MB->wdt, st->d_info);
+    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 0, st->d_info);
+    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 1, st->d_info);
+    mirror_mark_written(B);
+    B->sync = 1;
+    return CHAMELEON_SUCCESS;
+}
+
+} /* namespace chx */
+
+extern "C" {
+
+int CHAMELEON_dpotrf_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, RUNTIME_sequence_t *sequence,
+                                RUNTIME_request_t *request)
+{
+    int rc = check_posv_descs("CHAMELEON_dpotrf_Tile", A, nullptr, sequence, request);
+    if (rc != CHAMELEON_SUCCESS) return rc;
+    if (uplo != ChamUpper && uplo != ChamLower) {
+        error("CHAMELEON_dpotrf_Tile", "illegal value of uplo");
+        return request_fail(sequence, request, -1);
+    }
+    return potrf_async(uplo, A, sequence);
+}
+
+int CHAMELEON_dpotrs_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence,
+                                RUNTIME_request_t *request)
+{
+    int rc = check_posv_descs("CHAMELEON_dpotrs_Tile", A, B, sequence, request);
+    if (rc != CHAMELEON_SUCCESS) return rc;
+    if (uplo != ChamUpper && uplo != ChamLower) {
+        error("CHAMELEON_dpotrs_Tile", "illegal value of uplo");
+        return request_fail(sequence, request, -1);
+    }
+    rc = potrs_async(uplo, A, B, sequence);
+    return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
+}
+
+int CHAMELEON_dposv_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence,
+                               RUNTIME_request_t *request)
+{
+    int rc = check_posv_descs("CHAMELEON_dposv_Tile", A, B, sequence, request);
+    if (rc != CHAMELEON_SUCCESS) return rc;
+    if (uplo != ChamUpper && uplo != ChamLower) {
+        error("CHAMELEON_dposv_Tile", "illegal value of uplo");
+        return request_fail(sequence, request, -1);
+    }
+    potrf_async(uplo, A, sequence);
+    /* the solve kernels return at once when the device info flag is set,
+     * so a failed factorisation costs no host round trip here */
+    rc = potrs_async(uplo, A, B, sequence);
+    return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
+}
+
+static int sync_wrap(int (*fn)(RUNTIME_sequence_t *, RUNTIME_request_t *, void *), void *arg, CHAM_desc_t *A,
+                     CHAM_desc_t *B, const char *name)
+{
+    if (!ctx()) {
+        error(name, "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    RUNTIME_sequence_t *sequence = nullptr;
+    RUNTIME_request_t request = RUNTIME_REQUEST_INITIALIZER;
+    CHAMELEON_Sequence_Create(&sequence);
+    fn(sequence, &request, arg);
+    if (A) CHAMELEON_Desc_Flush(A, sequence);
+    if (B) CHAMELEON_Desc_Flush(B, sequence);
+    int status = sequence_wait_internal(sequence);
+    CHAMELEON_Sequence_Destroy(sequence);
+    return status;
+}
+
+struct PosvArgs { cham_uplo_t uplo; CHAM_desc_t *A, *B; };
+
+int CHAMELEON_dposv_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B)
+{
+    PosvArgs a = {uplo, A, B};
+    return sync_wrap([](RUNTIME_sequence_t *s, RUNTIME_request_t *r, void *p) {
+        PosvArgs *x = (PosvArgs *)p;
+        return CHAMELEON_dposv_Tile_Async(x->uplo, x->A, x->B, s, r);
+    }, &a, A, B, "CHAMELEON_dposv_Tile");
+}
+
+int CHAMELEON_dpotrf_Tile(cham_uplo_t uplo, CHAM_desc_t *A)
+{
+    PosvArgs a = {uplo, A, nullptr};
+    return sync_wrap([](RUNTIME_sequence_t *s, RUNTIME_request_t *r, void *p) {
+        PosvArgs *x = (PosvArgs *)p;
+        return CHAMELEON_dpotrf_Tile_Async(x->uplo, x->A, s, r);
+    }, &a, A, nullptr, "CHAMELEON_dpotrf_Tile");
+}
+
+int CHAMELEON_dpotrs_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B)
+{
+    PosvArgs a = {uplo, A, B};
+    return sync_wrap([](RUNTIME_sequence_t *s, RUNTIME_request_t *r, void *p) {
+        PosvArgs *x = (PosvArgs *)p;
+        return CHAMELEON_dpotrs_Tile_Async(x->uplo, x->A, x->B, s, r);
+    }, &a, nullptr, B, "CHAMELEON_dpotrs_Tile");
+}
+
+} /* extern "C" */

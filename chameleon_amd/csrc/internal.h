@@ -1,0 +1,129 @@
+/*
+ * internal.h -- library-private state of libchameleon_amd.
+ *
+ * Device mirror.  Every descriptor used on the GPU gets a mirror of its
+ * (sub)matrix in HBM: mt x nt tiles stored tile after tile in column-major
+ * tile order, each tile padded to ldt x wdt elements (ldt = mb rounded up to
+ * a multiple of 16, wdt likewise unless the matrix is a single tile column,
+ * where wdt = n rounded up to 16).  Padding is zero off the diagonal tiles
+ * and the identity on the padded diagonal of diagonal tiles, so the padded
+ * matrix has the same Cholesky factor as the real one with identity blocks
+ * in the holes; every kernel therefore works on full ldt x ldt tiles.
+ *
+ * A mirror is either in normal orientation (tile (m,n) holds A(m,n)) or
+ * transposed (tile (m,n) holds A(n,m)^T).  The Upper variants of the
+ * Cholesky path run the Lower kernels on a transposed mirror: for a
+ * symmetric A, the upper factor U satisfies U = L^T.
+ */
+#ifndef CHAMELEON_AMD_INTERNAL_H
+#define CHAMELEON_AMD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <vector>
+#include "chameleon.h"
+
+namespace chx {
+
+void fatal_hip(hipError_t e, const char *what, const char *file, int line);
+#define CHX_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) ::chx::fatal_hip(_e, #call, __FILE__, __LINE__); } while (0)
+
+void error(const char *func, const char *msg);
+void warning(const char *func, const char *msg);
+
+enum MirrorState { MIRROR_NONE = 0, MIRROR_NORMAL = 1, MIRROR_TRANS = 2 };
+
+struct Mirror {
+    void   *d = nullptr;       /* device tiles                              */
+    int     mt = 0, nt = 0;    /* tiles of the submatrix                    */
+    int     ldt = 0, wdt = 0;  /* padded tile rows / columns                */
+    size_t  tile_elems = 0;    /* ldt * wdt                                 */
+    size_t  bytes = 0;
+    int     eltsize = 8;
+    int     state = MIRROR_NONE;  /* content valid on device, orientation   */
+    bool    host_valid = true;    /* host copy up to date                   */
+    bool    diag_identity = false;/* padding of diagonal tiles is identity  */
+};
+
+struct Context {
+    int device = 0;
+    int nb = 384;                 /* CHAMELEON_TILE_SIZE default            */
+    int ib = 48;
+    int translation = ChamInPlace;
+    int lookahead = 1;
+    int autotuning = 1;
+    int warnings = 1;
+    int nworkers = 1;
+    hipStream_t s_main = nullptr;     /* critical path / default work     */
+    hipStream_t s_update = nullptr;   /* trailing updates (lookahead)      */
+    int        *d_info = nullptr;     /* device info flag (one int)        */
+    /* workspace reused across calls, grown on demand */
+    void  *ws = nullptr;
+    size_t ws_bytes = 0;
+    void  *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    std::vector<hipEvent_t> events;
+};
+
+Context *ctx();                       /* nullptr when not initialised */
+void *workspace(size_t bytes);        /* context workspace of >= bytes */
+
+Mirror *mirror_of(const CHAM_desc_t *A);
+/* Make the device mirror of A hold its current values in orientation
+ * `state` (MIRROR_NORMAL or MIRROR_TRANS); uploads from host if needed. */
+int mirror_acquire(CHAM_desc_t *A, int state, bool diag_identity, hipStream_t s);
+/* Allocate the mirror (without uploading) for a matrix that the next
+ * operation overwrites entirely. */
+int mirror_prepare_overwrite(CHAM_desc_t *A, int state, bool diag_identity);
+/* Mark device content as the latest (host stale). */
+void mirror_mark_written(CHAM_desc_t *A);
+/* Copy device content back to the host tiles (normal orientation). */
+int mirror_flush(CHAM_desc_t *A, hipStream_t s);
+void mirror_release(CHAM_desc_t *A);
+
+int desc_check(const CHAM_desc_t *A);
+int request_fail(RUNTIME_sequence_t *seq, RUNTIME_request_t *req, int status);
+
+/* sequences: device info slot for potrf of this sequence */
+struct SeqState {
+    int *d_info = nullptr;      /* device int, first failing minor        */
+    int *h_info = nullptr;      /* pinned host copy                       */
+    bool pending_info = false;
+};
+SeqState *seq_state(RUNTIME_sequence_t *seq);
+int sequence_wait_internal(RUNTIME_sequence_t *seq);
+
+/* ---- kernel launchers (kernels directory) ---- */
+namespace k {
+/* util.hip */
+void fix_padding(void *d, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N, int eltsize,
+                 bool diag_identity, hipStream_t s);
+void transpose_tiles_inplace(double *d, int nt, int ldt, hipStream_t s);
+void plgsy(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N, double bump, int uplo_skip,
+           int bigM, int m0, int n0, unsigned long long seed, int trans, bool diag_identity, hipStream_t s);
+void plrnt(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N, int bigM, int m0, int n0,
+           unsigned long long seed, int trans, hipStream_t s);
+void set_identity(double *d, int ntiles, int ldt, hipStream_t s);
+void norm_partials(const double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N, int uplo,
+                   int trans, double *colsum, double *rowsum, double *w, hipStream_t s);
+void lacpy_tiles(const double *src, double *dst, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N,
+                 int uplo, hipStream_t s);
+/* chol.hip */
+void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, hipStream_t s);
+void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int ldt, const double *Lkk,
+                            const double *dinv, const int *info, hipStream_t s);
+void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, int ldt, const double *L0,
+                                    long long lstride, const double *dinv0, long long dstride, const int *info,
+                                    hipStream_t s);
+void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s);
+void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
+void gemm_batched(int transA, int M, int N, int K, double alpha, const double *A, int lda, long long strideA,
+                  const double *B, int ldb, long long strideB, double beta, double *C, int ldc, long long strideC,
+                  int batch, double *copy_dst, const int *info, hipStream_t s);
+}
+
+} /* namespace chx */
+
+#endif

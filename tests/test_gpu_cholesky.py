@@ -1,0 +1,296 @@
+"""GPU parity of the FP64 Cholesky path, through the C ABI.
+
+Every case compares libchameleon_amd on the MI355X with the oracle (the C
+restatement of Chameleon's CPU path, see oracle/cham_oracle.c) on the same
+seeded dplgsy/dplrnt inputs, and applies Chameleon's own acceptance checks
+(testing/testing_zcheck_facto.c, threshold 60).  Tolerances:
+
+* generators: bit-exact;
+* factor: relative Frobenius distance to the oracle factor <= 1e-12;
+* solution: relative Frobenius distance to the oracle solution <= 1e-11
+  (it is the factor's error amplified by cond(A), which is O(1) for the
+  diagonally bumped generator);
+* non-SPD inputs: the same `info` as LAPACK dpotrf.
+
+The parameter grids are those of testing/input/{potrf,potrs,posv}.in
+(nb 16/17, odd n, lda > n, nrhs > nb) plus the large tile sizes of the
+MI355X path.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+L_, U_, UL_ = 122, 121, 123
+TOL_FACTOR = 1e-12
+TOL_SOLVE = 1e-11
+
+
+@pytest.fixture(scope="module")
+def ch():
+    import chameleon_amd as ch
+    ch.init()
+    yield ch
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def reference_solution(uplo, n, nrhs, nb, lda=None, ldb=None):
+    """Oracle run of CHAMELEON_dposv on the generator inputs."""
+    lda = lda or n
+    ldb = ldb or n
+    A = oracle.dplgsy(float(n), uplo, n, 51, lda=lda, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=ldb, nb=nb)
+    info = oracle.dposv(uplo, A, n, B, nrhs, nb)
+    return A, B, info
+
+
+# ----------------------------------------------------------------- generators
+@pytest.mark.parametrize("n,nb,lda", [(15, 16, 15), (37, 17, 41), (100, 32, 100), (1000, 128, 1003), (700, 320, 700)])
+@pytest.mark.parametrize("uplo", [L_, U_, UL_])
+def test_dplgsy_bitexact(ch, n, nb, lda, uplo):
+    say("dplgsy", n, nb, lda, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = np.full((lda, n), 3.25, order="F")
+    ch.check(ch.lib.CHAMELEON_dplgsy(float(n), uplo, n, _p(A), lda, 51))
+    R = np.full((lda, n), 3.25, order="F")
+    oracle.dplgsy(float(n), uplo, n, 51, lda=lda, nb=nb, out=R)
+    assert np.array_equal(A, R)
+
+
+@pytest.mark.parametrize("m,n,nb", [(37, 13, 17), (16, 22, 16), (1000, 64, 320), (333, 777, 128)])
+def test_dplrnt_bitexact(ch, m, n, nb):
+    say("dplrnt", m, n, nb)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = np.zeros((m, n), order="F")
+    ch.check(ch.lib.CHAMELEON_dplrnt(m, n, _p(A), m, 5673))
+    assert np.array_equal(A, oracle.dplrnt(m, n, 5673, nb=nb))
+
+
+# ------------------------------------------------------- posv.in grid (tile API)
+POSV_GRID = [(nb, n, nrhs) for nb in (16, 17) for n in (15, 21, 35) for nrhs in (1, 13, 22, 33)]
+
+
+@pytest.mark.parametrize("nb,n,nrhs", POSV_GRID)
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_posv_tile_grid(ch, nb, n, nrhs, uplo):
+    lda, ldb = 37, 39
+    say("posv_tile", nb, n, nrhs, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), uplo, n, 51, lda=lda, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=ldb, nb=nb)
+    A_in = A.copy(order="F")
+    dA = ch.Desc.from_array(A, nb, n, n)
+    dB = ch.Desc.from_array(B, nb, n, nrhs)
+    try:
+        assert ch.lib.CHAMELEON_dposv_Tile(uplo, dA.ptr, dB.ptr) == 0
+    finally:
+        dA.destroy()
+        dB.destroy()
+    Ar, Br, info = reference_solution(uplo, n, nrhs, nb, lda, ldb)
+    assert info == 0
+    tri = np.tril if uplo == L_ else np.triu
+    assert oracle.rel_frobenius(tri(A[:n, :n]), tri(Ar[:n, :n])) <= TOL_FACTOR
+    assert oracle.rel_frobenius(B[:n], Br[:n]) <= TOL_SOLVE
+    # the other triangle and the rows beyond n are untouched
+    other = np.triu if uplo == L_ else np.tril
+    k = 1 if uplo == L_ else -1
+    assert np.array_equal(other(A[:n, :n], k), other(A_in[:n, :n], k))
+    assert np.array_equal(A[n:], A_in[n:])
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    assert oracle.check_factorization(A0, A[:n, :n], uplo)[0]
+    B0 = oracle.dplrnt(n, nrhs, 5673)
+    assert oracle.check_solve(A0, B[:n], B0)[0]
+
+
+@pytest.mark.parametrize("nb,n", [(nb, n) for nb in (16, 17) for n in (15, 19, 37)])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_potrf_then_potrs_ccrb(ch, nb, n, uplo):
+    """Library-allocated (CCRB) descriptors filled through Lap2Desc, as the
+    reference testing drivers do, potrf.in/potrs.in sizes (lda 41, ldb 43)."""
+    say("potrf_potrs_ccrb", nb, n, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    for nrhs in (1, 13, 21, 35):
+        A = oracle.dplgsy(float(n), UL_, n, 51, lda=41, nb=nb)
+        B = oracle.dplrnt(n, nrhs, 5673, lda=43, nb=nb)
+        with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB:
+            ch.check(ch.lib.CHAMELEON_Lap2Desc(UL_, _p(A), 41, dA.ptr))
+            ch.check(ch.lib.CHAMELEON_Lap2Desc(UL_, _p(B), 43, dB.ptr))
+            assert ch.lib.CHAMELEON_dpotrf_Tile(uplo, dA.ptr) == 0
+            assert ch.lib.CHAMELEON_dpotrs_Tile(uplo, dA.ptr, dB.ptr) == 0
+            F = dA.to_array(n, n)
+            X = dB.to_array(n, nrhs)
+        Ar, Br, info = reference_solution(uplo, n, nrhs, nb, 41, 43)
+        tri = np.tril if uplo == L_ else np.triu
+        assert oracle.rel_frobenius(tri(F), tri(Ar[:n, :n])) <= TOL_FACTOR
+        assert oracle.rel_frobenius(X, Br[:n]) <= TOL_SOLVE
+
+
+# ------------------------------------------------------------- LAPACK layout
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_lapack_layout_and_lapacke(ch, uplo):
+    n, nrhs, nb = 150, 7, 64
+    say("lapack", uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    Ar, Br, _ = reference_solution(uplo, n, nrhs, nb, n + 3, n + 5)
+    tri = np.tril if uplo == L_ else np.triu
+    ch_u = b"L" if uplo == L_ else b"U"
+    # CHAMELEON_dposv
+    A = oracle.dplgsy(float(n), uplo, n, 51, lda=n + 3, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=n + 5, nb=nb)
+    assert ch.lib.CHAMELEON_dposv(uplo, n, nrhs, _p(A), n + 3, _p(B), n + 5) == 0
+    assert oracle.rel_frobenius(tri(A[:n]), tri(Ar[:n])) <= TOL_FACTOR
+    assert oracle.rel_frobenius(B[:n], Br[:n]) <= TOL_SOLVE
+    # CHAMELEON_dpotrf + CHAMELEON_dpotrs
+    A = oracle.dplgsy(float(n), uplo, n, 51, lda=n + 3, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=n + 5, nb=nb)
+    assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n + 3) == 0
+    assert ch.lib.CHAMELEON_dpotrs(uplo, n, nrhs, _p(A), n + 3, _p(B), n + 5) == 0
+    assert oracle.rel_frobenius(B[:n], Br[:n]) <= TOL_SOLVE
+    # LAPACKE
+    A = oracle.dplgsy(float(n), uplo, n, 51, lda=n + 3, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=n + 5, nb=nb)
+    assert ch.lib.CHAMELEON_lapacke_dposv(ch.COL_MAJOR, ch_u, n, nrhs, _p(A), n + 3, _p(B), n + 5) == 0
+    assert oracle.rel_frobenius(B[:n], Br[:n]) <= TOL_SOLVE
+    assert ch.lib.CHAMELEON_lapacke_dposv(101, ch_u, n, nrhs, _p(A), n + 3, _p(B), n + 5) == -1
+    # Fortran
+    A = oracle.dplgsy(float(n), uplo, n, 51, lda=n + 3, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=n + 5, nb=nb)
+    c_u = ctypes.create_string_buffer(ch_u)
+    iv = lambda v: ctypes.byref(ctypes.c_int(v))
+    info = ctypes.c_int(-99)
+    ch.lib.chameleon_lapack_dposv_(c_u, iv(n), iv(nrhs), _p(A), iv(n + 3), _p(B), iv(n + 5), ctypes.byref(info))
+    assert info.value == 0
+    assert oracle.rel_frobenius(B[:n], Br[:n]) <= TOL_SOLVE
+
+
+def test_argument_errors(ch):
+    say("argument errors")
+    A = np.zeros((4, 4), order="F")
+    L = ch.lib
+    assert L.CHAMELEON_dposv(99, 4, 1, _p(A), 4, _p(A), 4) == -1
+    assert L.CHAMELEON_dposv(L_, -1, 1, _p(A), 4, _p(A), 4) == -2
+    assert L.CHAMELEON_dposv(L_, 4, -1, _p(A), 4, _p(A), 4) == -3
+    assert L.CHAMELEON_dposv(L_, 4, 1, _p(A), 3, _p(A), 4) == -5
+    assert L.CHAMELEON_dposv(L_, 4, 1, _p(A), 4, _p(A), 3) == -7
+    assert L.CHAMELEON_dpotrf(L_, 4, _p(A), 3) == -4
+    assert L.CHAMELEON_dposv(L_, 0, 1, _p(A), 1, _p(A), 1) == 0
+    # non-square tiles are refused by the tile interface
+    p = ctypes.c_void_p()
+    assert L.CHAMELEON_Desc_Create(ctypes.byref(p), _p(A), ch.REAL_DOUBLE, 2, 4, 8, 4, 4, 0, 0, 4, 4, 1, 1) == 0
+    assert L.CHAMELEON_dpotrf_Tile(L_, p) == ch.ERR_ILLEGAL_VALUE
+    L.CHAMELEON_Desc_Destroy(ctypes.byref(p))
+
+
+# ------------------------------------------------------------------ non-SPD
+@pytest.mark.parametrize("key,n,bump,seed", [("nonspd_n21_info", 21, 0.0, 3), ("nonspd_n40_b2_info", 40, 2.0, 9)])
+@pytest.mark.parametrize("nb", [4, 16, 17])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_non_spd_info(ch, golden, key, n, bump, seed, nb, uplo):
+    say("non-spd", key, nb, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(bump, UL_, n, seed)
+    info = ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n)
+    assert info == int(golden[key][0])
+    A = oracle.dplgsy(bump, UL_, n, seed)
+    B = np.ones((n, 2), order="F")
+    assert ch.lib.CHAMELEON_dposv(uplo, n, 2, _p(A), n, _p(B), n) == int(golden[key][0])
+
+
+# ---------------------------------------------------------- large tile sizes
+@pytest.mark.parametrize("n,nb,nrhs", [(1000, 128, 16), (1000, 320, 64), (2048, 256, 64), (2100, 384, 64),
+                                       (2048, 512, 1), (1536, 192, 3)])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_large_tiles_vs_oracle(ch, n, nb, nrhs, uplo):
+    say("large", n, nb, nrhs, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    assert ch.lib.CHAMELEON_dposv(uplo, n, nrhs, _p(A), n, _p(B), n) == 0
+    oracle.use_blas(True)
+    try:
+        Ar, Br, info = reference_solution(uplo, n, nrhs, nb)
+    finally:
+        oracle.use_blas(False)
+    tri = np.tril if uplo == L_ else np.triu
+    assert oracle.rel_frobenius(tri(A), tri(Ar)) <= TOL_FACTOR
+    assert oracle.rel_frobenius(B, Br) <= TOL_SOLVE
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    assert oracle.check_factorization(A0, A, uplo)[0]
+    assert oracle.check_solve(A0, B, oracle.dplrnt(n, nrhs, 5673))[0]
+
+
+# -------------------------------------------------- device-resident workflow
+def test_async_device_resident_matches_sync(ch):
+    n, nb, nrhs = 1024, 256, 64
+    say("async", n, nb)
+    ch.set_param(ch.TILE_SIZE, nb)
+    L = ch.lib
+    with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB:
+        seq = ch.Sequence()
+        assert L.CHAMELEON_dplgsy_Tile_Async(float(n), UL_, dA.ptr, 51, seq.ptr, seq.req) == 0
+        assert L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673, seq.ptr, seq.req) == 0
+        assert L.CHAMELEON_dposv_Tile_Async(L_, dA.ptr, dB.ptr, seq.ptr, seq.req) == 0
+        assert L.CHAMELEON_Desc_Flush(dA.ptr, seq.ptr) == 0
+        assert L.CHAMELEON_Desc_Flush(dB.ptr, seq.ptr) == 0
+        assert seq.wait() == 0
+        seq.destroy()
+        F = dA.to_array(n, n)
+        X = dB.to_array(n, nrhs)
+    A = oracle.dplgsy(float(n), UL_, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    assert ch.lib.CHAMELEON_dposv(L_, n, nrhs, _p(A), n, _p(B), n) == 0
+    assert np.array_equal(np.tril(F), np.tril(A))
+    assert np.array_equal(X, B)
+
+
+def test_lookahead_does_not_change_bits(ch):
+    n, nb = 1500, 128
+    say("lookahead", n, nb)
+    ch.set_param(ch.TILE_SIZE, nb)
+    out = []
+    for la in (1, 0):
+        ch.set_param(ch.LOOKAHEAD, la)
+        A = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+        assert ch.lib.CHAMELEON_dpotrf(L_, n, _p(A), n) == 0
+        out.append(np.tril(A))
+    ch.set_param(ch.LOOKAHEAD, 1)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_upper_is_transpose_of_lower(ch):
+    n, nb = 600, 64
+    say("upper vs lower", n, nb)
+    ch.set_param(ch.TILE_SIZE, nb)
+    AL = oracle.dplgsy(float(n), UL_, n, 51)
+    AU = AL.copy(order="F")
+    assert ch.lib.CHAMELEON_dpotrf(L_, n, _p(AL), n) == 0
+    assert ch.lib.CHAMELEON_dpotrf(U_, n, _p(AU), n) == 0
+    assert np.array_equal(np.tril(AL), np.triu(AU).T)
+
+
+def test_norms(ch):
+    n, nb = 300, 64
+    say("norms")
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), UL_, n, 51)
+    G = oracle.dplrnt(n, 70, 3)
+    L = ch.lib
+    assert np.isclose(L.CHAMELEON_dlange(ch.ONE_NORM, n, 70, _p(G), n), np.abs(G).sum(0).max(), rtol=1e-13)
+    assert np.isclose(L.CHAMELEON_dlange(ch.INF_NORM, n, 70, _p(G), n), np.abs(G).sum(1).max(), rtol=1e-13)
+    assert L.CHAMELEON_dlange(ch.MAX_NORM, n, 70, _p(G), n) == np.abs(G).max()
+    assert np.isclose(L.CHAMELEON_dlange(ch.FROBENIUS_NORM, n, 70, _p(G), n), np.linalg.norm(G), rtol=1e-13)
+    for uplo in (L_, U_):
+        assert np.isclose(L.CHAMELEON_dlansy(ch.ONE_NORM, uplo, n, _p(A), n), oracle.dlansy_one(A, uplo), rtol=1e-13)
