@@ -25,6 +25,7 @@
  * (forward) or above (backward) in one strided batched GEMM launch.
  */
 #include "internal.h"
+#include "chameleon_amd.h"
 
 #include <algorithm>
 #include <cstring>
@@ -56,19 +57,46 @@ static void potrf_run(const PotrfPlan &p)
     hipStream_t sm = c->s_main, su = c->s_update;
     const int NT = p.NT, ldt = p.ldt;
     const size_t ts = (size_t)ldt * ldt;
-    const int nblk = ldt / 16;
-    double *dinv = (double *)workspace((size_t)NT * nblk * 256 * sizeof(double));
+    /* explicit inverses of the 64 x 64 diagonal blocks, per diagonal tile */
+    const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
+    double *dinv = (double *)workspace((size_t)NT * dper * sizeof(double));
     auto tile = [&](int m, int n) { return p.A + ((size_t)n * NT + m) * ts; };
+    /* algorithmic flops (include/chameleon/flops.h) of each launch, for
+     * the kernel statistics */
+    auto tsz = [&](int k) { return (double)((k == NT - 1) ? p.nvalid_last : p.nb); };
+    const double nbd = p.nb;
+    auto update_flops = [&](int k, int n0, int n1) {
+        double f = 0.0;
+        for (int n = n0; n < n1; ++n) {
+            f += nbd * tsz(n) * (tsz(n) + 1.0);
+            for (int m = n + 1; m < NT; ++m) f += 2.0 * tsz(m) * tsz(n) * nbd;
+        }
+        (void)k;
+        return f;
+    };
+    auto update = [&](int k, int n0, int n1, hipStream_t s) {
+        if (n0 >= NT) return;
+        timed(CHAMAMD_KERNEL_UPDATE, update_flops(k, n0, n1), s,
+              [&] { k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s); });
+    };
     auto panel = [&](int k) {
         const int nvalid = (k == NT - 1) ? p.nvalid_last : p.nb;
-        double *dk = dinv + (size_t)k * nblk * 256;
-        k::potrf_diag(tile(k, k), ldt, dk, p.nb * k, nvalid, p.d_info, sm);
-        if (k + 1 < NT) k::trsm_right_lower_trans(tile(k + 1, k), NT - k - 1, ts, ldt, tile(k, k), dk, p.d_info, sm);
+        double *dk = dinv + (size_t)k * dper;
+        const double nk = tsz(k);
+        timed(CHAMAMD_KERNEL_POTRF, nk * (nk * nk / 3.0 + nk / 2.0 + 1.0 / 6.0), sm,
+              [&] { k::potrf_diag(tile(k, k), ldt, dk, p.nb * k, nvalid, p.d_info, sm); });
+        if (k + 1 < NT) {
+            double f = 0.0;
+            for (int m = k + 1; m < NT; ++m) f += tsz(m) * nbd * nbd;
+            timed(CHAMAMD_KERNEL_TRSM, f, sm, [&] {
+                k::trsm_right_lower_trans(tile(k + 1, k), NT - k - 1, ts, ldt, tile(k, k), dk, p.d_info, sm);
+            });
+        }
     };
     if (c->lookahead <= 0 || NT <= 2) {
         for (int k = 0; k < NT; ++k) {
             panel(k);
-            k::trailing_update(p.A, NT, ldt, k, k + 1, NT, NT, p.d_info, sm);
+            update(k, k + 1, NT, sm);
         }
         return;
     }
@@ -81,11 +109,11 @@ static void potrf_run(const PotrfPlan &p)
     for (int k = 0; k < NT - 1; ++k) {
         /* update stream: columns k+2.. with panel k */
         CHX_HIP(hipStreamWaitEvent(su, event(2 * k), 0));
-        if (k + 2 < NT) k::trailing_update(p.A, NT, ldt, k, k + 2, NT, NT, p.d_info, su);
+        if (k + 2 < NT) update(k, k + 2, NT, su);
         CHX_HIP(hipEventRecord(event(2 * k + 1), su));
         /* critical stream: column k+1, then panel k+1 */
         if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
-        k::trailing_update(p.A, NT, ldt, k, k + 1, k + 2, NT, p.d_info, sm);
+        update(k, k + 1, k + 2, sm);
         panel(k + 1);
         CHX_HIP(hipEventRecord(event(2 * (k + 1)), sm));
     }
@@ -97,8 +125,9 @@ static void potrf_run(const PotrfPlan &p)
  * every tile column of B with the Lower factor in A; Tinv holds, for every
  * diagonal tile k, Linv_k^T (upper triangular). */
 static void solve_run(double *A, int NT, int ldt, const double *Tinv, double *B, int Bnt, int wdt, int trans,
-                      int *d_info)
+                      int *d_info, int nb, int nvalid_last, int nrhs, int bnb)
 {
+    auto tsz = [&](int k) { return (double)((k == NT - 1) ? nvalid_last : nb); };
     Context *c = ctx();
     hipStream_t sm = c->s_main;
     const size_t ts = (size_t)ldt * ldt;
@@ -112,18 +141,24 @@ static void solve_run(double *A, int NT, int ldt, const double *Tinv, double *B,
             const double *Tk = Tinv + (size_t)k * ts;
             double *Bk = Bc + (size_t)k * bts;
             /* W = op(Linv_k) B_k: forward uses Linv = Tk^T, backward Linv^T = Tk */
-            k::gemm_batched(trans ? 0 : 1, ldt, wdt, ldt, 1.0, Tk, ldt, 0, Bk, ldt, 0, 0.0, Wb, ldt, 0, 1, nullptr,
-                            d_info, sm);
-            if (!trans) {
-                /* B_m -= A(m,k) W for m > k, and B_k <- W */
-                const int cnt = NT - k - 1;
-                k::gemm_batched(0, ldt, wdt, ldt, -1.0, A + ((size_t)k * NT + k + 1) * ts, ldt, (long long)ts, Wb,
-                                ldt, 0, 1.0, Bc + (size_t)(k + 1) * bts, ldt, (long long)bts, cnt, Bk, d_info, sm);
-            } else {
-                /* B_m -= A(k,m)^T W for m < k, and B_k <- W */
-                k::gemm_batched(1, ldt, wdt, ldt, -1.0, A + (size_t)k * ts, ldt, (long long)NT * ts, Wb, ldt, 0,
-                                1.0, Bc, ldt, (long long)bts, k, Bk, d_info, sm);
-            }
+            const int cnt = trans ? k : NT - k - 1;
+            const double w = (double)((bn == Bnt - 1) ? nrhs - bn * bnb : bnb);
+            double fl = tsz(k) * tsz(k) * w;
+            for (int m = (trans ? 0 : k + 1); m < (trans ? k : NT); ++m) fl += 2.0 * tsz(m) * tsz(k) * w;
+            timed(CHAMAMD_KERNEL_SOLVE, fl, sm, [&] {
+                k::gemm_batched(trans ? 0 : 1, ldt, wdt, ldt, 1.0, Tk, ldt, 0, Bk, ldt, 0, 0.0, Wb, ldt, 0, 1, nullptr,
+                                d_info, sm);
+                if (!trans) {
+                    /* B_m -= A(m,k) W for m > k, and B_k <- W */
+                    k::gemm_batched(0, ldt, wdt, ldt, -1.0, A + ((size_t)k * NT + k + 1) * ts, ldt, (long long)ts,
+                                    Wb, ldt, 0, 1.0, Bc + (size_t)(k + 1) * bts, ldt, (long long)bts, cnt, Bk, d_info,
+                                    sm);
+                } else {
+                    /* B_m -= A(k,m)^T W for m < k, and B_k <- W */
+                    k::gemm_batched(1, ldt, wdt, ldt, -1.0, A + (size_t)k * ts, ldt, (long long)NT * ts, Wb, ldt, 0,
+                                    1.0, Bc, ldt, (long long)bts, k, Bk, d_info, sm);
+                }
+            });
         }
     }
 }
@@ -135,8 +170,8 @@ static double *form_inverses(double *A, int NT, int ldt, int wdt, int *d_info)
     Context *c = ctx();
     hipStream_t sm = c->s_main;
     const size_t ts = (size_t)ldt * ldt;
-    const int nblk = ldt / 16;
-    const size_t dinv_elems = (size_t)NT * nblk * 256;
+    const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
+    const size_t dinv_elems = (size_t)NT * dper;
     const size_t need = ((size_t)NT * ts + (size_t)ldt * wdt + dinv_elems) * sizeof(double);
     double *ws = (double *)workspace(need);
     double *Tinv = ws;
@@ -144,7 +179,7 @@ static double *form_inverses(double *A, int NT, int ldt, int wdt, int *d_info)
     k::dinv_all(A, NT, (size_t)(NT + 1) * ts, ldt, dinv, sm);
     k::set_identity(Tinv, NT, ldt, sm);
     k::trsm_right_lower_trans_batched(Tinv, NT, (long long)ts, ldt, A, (long long)(NT + 1) * ts, dinv,
-                                      (long long)nblk * 256, d_info, sm);
+                                      (long long)dper, d_info, sm);
     return Tinv;
 }
 
@@ -226,8 +261,11 @@ int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequen
         return CHAMELEON_ERR_ILLEGAL_VALUE;
     }
     double *Tinv = form_inverses((double *)MA->d, MA->mt, MA->ldt, MB->wdt, st->d_info);
-    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 0, st->d_info);
-    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 1, st->d_info);
+    const int nvl = A->m - (MA->mt - 1) * A->mb;
+    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 0, st->d_info, A->mb, nvl,
+              B->n, B->nb);
+    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 1, st->d_info, A->mb, nvl,
+              B->n, B->nb);
     mirror_mark_written(B);
     B->sync = 1;
     return CHAMELEON_SUCCESS;

@@ -65,9 +65,35 @@ struct Context {
     void  *pinned = nullptr;
     size_t pinned_bytes = 0;
     std::vector<hipEvent_t> events;
+    /* kernel statistics (CHAMELEON_GENERATE_STATS) */
+    bool stats = false;
+    struct Pending { int cls; hipEvent_t a, b; double flops; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> timing_pool;
+    long long st_launches[4] = {0, 0, 0, 0};
+    double st_ms[4] = {0, 0, 0, 0};
+    double st_flops[4] = {0, 0, 0, 0};
 };
 
 Context *ctx();                       /* nullptr when not initialised */
+inline Context *ctx_ptr() { return ctx(); }
+
+/* Bracket the launches issued by `body` on stream s with timing events
+ * when statistics are enabled. */
+hipEvent_t timing_event();
+void stats_fold();
+template <typename F>
+inline void timed(int cls, double flops, hipStream_t s, F body)
+{
+    Context *c = ctx_ptr();
+    if (!c->stats) { body(); return; }
+    hipEvent_t a = timing_event(), b = timing_event();
+    (void)hipEventRecord(a, s);
+    body();
+    (void)hipEventRecord(b, s);
+    c->pending.push_back({cls, a, b, flops});
+}
+
 void *workspace(size_t bytes);        /* context workspace of >= bytes */
 
 Mirror *mirror_of(const CHAM_desc_t *A);

@@ -1,0 +1,432 @@
+/*
+ * panel.hip -- the critical-path kernels of the tiled Cholesky on gfx950:
+ * POTRF of a diagonal tile (the tile kernel behind INSERT_TASK_dpotrf,
+ * coreblas/compute/core_zpotrf.c:75-81, called at compute/pzpotrf.c:90-94)
+ * and the panel TRSM (INSERT_TASK_dtrsm(Right, Lower, Trans, NonUnit),
+ * pzpotrf.c:99-105), plus the 64 x 64 diagonal-block inverses they share.
+ *
+ * Both work on 64-column blocks.  POTRF factors a 64-column block of the
+ * diagonal tile with a 16-column inner loop (16 x 16 blocks factored in
+ * registers by one wavefront), then builds the explicit inverse of the
+ * 64 x 64 diagonal block from the 16 x 16 inverses; TRSM then needs one
+ * 64-column MFMA product per block instead of a 16-column substitution
+ * chain.  The inverses are stored as 64 x 64 column-major blocks, ldt/64
+ * (rounded up) per diagonal tile.
+ */
+#include "../internal.h"
+#include "mfma.h"
+
+namespace chx {
+namespace k {
+
+#define OB 64       /* outer block width */
+#define OBP 65      /* padded LDS row */
+
+/* ------------------------------------------------------------------ */
+/* acc[f] += X[0:16, 0:K] * Y[16f:16f+16, 0:K]^T for f < NF, one wavefront;
+ * X, Y column-major with leading dimension ld, K a multiple of 16, loads
+ * pipelined one 16-deep slice ahead.  Formed as D = Y X^T: lane l holds
+ * (row l%16, col 16f + l/16 + 4s) in slot s of acc[f].                    */
+template <int NF>
+__device__ __forceinline__ void wave_gemm_nt(const double *X, const double *Y, int ld, int K, int lane, v4d *acc)
+{
+    const int li = lane & 15, lk = lane >> 4;
+    if (K <= 0) return;
+    const double *px = X + li + (size_t)lk * ld;
+    const double *py = Y + li + (size_t)lk * ld;
+    double x[4], y[NF][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        x[q] = px[(size_t)(4 * q) * ld];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) y[f][q] = py[16 * f + (size_t)(4 * q) * ld];
+    }
+    for (int p = 0; p < K; p += 16) {
+        double xn[4], yn[NF][4];
+        const bool more = p + 16 < K;
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                xn[q] = px[(size_t)(p + 16 + 4 * q) * ld];
+#pragma unroll
+                for (int f = 0; f < NF; ++f) yn[f][q] = py[16 * f + (size_t)(p + 16 + 4 * q) * ld];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int f = 0; f < NF; ++f) acc[f] = mfma_f64(y[f][q], x[q], acc[f]);
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                x[q] = xn[q];
+#pragma unroll
+                for (int f = 0; f < NF; ++f) y[f][q] = yn[f][q];
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void wave_gemm_nt_dyn(int nf, const double *X, const double *Y, int ld, int K, int lane,
+                                                 v4d *acc)
+{
+    switch (nf) {
+    case 1: wave_gemm_nt<1>(X, Y, ld, K, lane, acc); break;
+    case 2: wave_gemm_nt<2>(X, Y, ld, K, lane, acc); break;
+    case 3: wave_gemm_nt<3>(X, Y, ld, K, lane, acc); break;
+    default: wave_gemm_nt<4>(X, Y, ld, K, lane, acc); break;
+    }
+}
+
+/* Cholesky of a 16 x 16 block by one wavefront: lane i (of each 16-lane
+ * group; the four groups compute the same thing) holds row i in a[];
+ * pivots and column entries travel by lane shuffles.  On return a[] holds
+ * row i of L.  `fail` is the 1-based index of the first non-positive pivot
+ * (uniform), 0 if none. */
+__device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fail)
+{
+    const int i = lane & 15;
+    fail = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double d = __shfl(a[j], j, 16);
+        if (fail == 0 && !(d > 0.0)) fail = j + 1;
+        const double sd = sqrt(d);
+        const double lij = (i >= j) ? a[j] / sd : 0.0;
+        if (i >= j) a[j] = (i == j) ? sd : lij;
+#pragma unroll
+        for (int k = j + 1; k < 16; ++k) {
+            const double lkj = __shfl(lij, k, 16);
+            if (i > j && k <= i) a[k] -= lij * lkj;
+        }
+    }
+}
+
+/* Column (lane % 16) of the inverse of the lower triangular 16 x 16 block
+ * L(r,p) = Ls[r * lds + p], by forward substitution. */
+__device__ __forceinline__ void block16_inverse_col(const double *Ls, int lds, int lane, double (&col)[16])
+{
+    const int jc = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        double s = (r == jc) ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = 0; p < r; ++p) s -= Ls[r * lds + p] * col[p];
+        col[r] = (r < jc) ? 0.0 : s / Ls[r * lds + r];
+    }
+}
+
+/* Inverse Lv of the lower triangular W x W block Ld (W = 16 nfb), given
+ * the inverses Lb[I] of its 16 x 16 diagonal blocks, by block rows:
+ *   Lv(I,I) = Lb(I),  Lv(I,J) = -Lb(I) sum_{J<=P<I} Ld(I,P) Lv(P,J).
+ * Called by the whole workgroup (barriers inside); wavefront J computes
+ * block column J. */
+__device__ void assemble_inverse(const double (*Ld)[OBP], const double (*Lb)[16][17], double (*Lv)[OBP], int nfb,
+                                 int wave, int lane)
+{
+    const int li = lane & 15, lk = lane >> 4;
+    /* diagonal blocks and zero upper blocks */
+    for (int e = threadIdx.x; e < OB * OB; e += blockDim.x) {
+        const int r = e / OB, c = e % OB;
+        const int I = r >> 4, J = c >> 4;
+        Lv[r][c] = (I == J && I < nfb) ? Lb[I][r & 15][c & 15] : 0.0;
+    }
+    __syncthreads();
+    for (int I = 1; I < nfb; ++I) {
+        const int J = wave;
+        if (J < I) {
+            v4d S = zero4d();
+            for (int P = J; P < I; ++P) {
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    S = mfma_f64(Ld[16 * I + li][16 * P + 4 * kk + lk], Lv[16 * P + 4 * kk + lk][16 * J + li], S);
+            }
+            /* slot kk of S holds S(4kk + lk, li): the B-operand layout */
+            v4d R = zero4d();
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) R = mfma_f64(Lb[I][li][4 * kk + lk], S[kk], R);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) Lv[16 * I + lk + 4 * s][16 * J + li] = -R[s];
+        }
+        __syncthreads();
+    }
+}
+
+/* ================================================================== */
+/* POTRF of one diagonal tile, one workgroup of 8 wavefronts.
+ * For each 64-column block C (left-looking):
+ *   (a) rows [C, ldt) of the block -= T[., 0:C] T[C:C+64, 0:C]^T   (MFMA)
+ *   (b) factor the 64 x 64 diagonal block in LDS by 16-column steps
+ *   (c) build its inverse, store it in dinv
+ *   (d) rows below: X = P Linv^T                                  (MFMA)
+ * The strictly upper triangle of the tile is never written.  info:
+ * iinfo + (1-based column of the first non-positive pivot), only for
+ * pivots inside the valid part of the tile (nvalid columns).            */
+#define POTRF_THREADS 512
+#define POTRF_WAVES 8
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(double *T, int ldt, double *dinv, int iinfo,
+                                                                    int nvalid, int *info)
+{
+    __shared__ double Ld[OB][OBP];
+    __shared__ double Lv[OB][OBP];
+    __shared__ double Lb[4][16][17];
+    __shared__ int s_fail;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    if (*info != 0) return;
+    if (tid == 0) s_fail = 0;
+    __syncthreads();
+
+    for (int C = 0; C < ldt; C += OB) {
+        const int W = min(OB, ldt - C), nfb = W >> 4;
+        /* (a) */
+        if (C > 0) {
+            const int nrb = (ldt - C) >> 4;
+            for (int rb = wave; rb < nrb; rb += POTRF_WAVES) {
+                const int r0 = C + rb * 16;
+                v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+                wave_gemm_nt_dyn(nfb, T + r0, T + C, ldt, C, lane, acc);
+                for (int f = 0; f < nfb; ++f)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int r = r0 + li, c = C + 16 * f + lk + 4 * s;
+                        if (r >= c) T[r + (size_t)c * ldt] -= acc[f][s];
+                    }
+            }
+            __syncthreads();
+        }
+        /* (b) diagonal block into LDS (lower part; zeros above) */
+        for (int e = tid; e < OB * OB; e += POTRF_THREADS) {
+            const int r = e % OB, c = e / OB;
+            Ld[r][c] = (r < W && c < W && r >= c) ? T[(C + r) + (size_t)(C + c) * ldt] : 0.0;
+        }
+        __syncthreads();
+        for (int cb = 0; cb < nfb; ++cb) {
+            const int c0 = 16 * cb;
+            /* (b1) columns [c0, c0+16) of rows [c0, W) -= Ld[., 0:c0] Ld[c0.., 0:c0]^T */
+            if (cb > 0) {
+                const int rb = wave;
+                if (rb < nfb - cb) {
+                    const int r0 = c0 + 16 * rb;
+                    v4d acc = zero4d();
+                    for (int p = 0; p < c0; p += 4)
+                        acc = mfma_f64(Ld[c0 + li][p + lk], Ld[r0 + li][p + lk], acc);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int c = lk + 4 * s;
+                        if (rb > 0 || li >= c) Ld[r0 + li][c0 + c] -= acc[s];
+                    }
+                }
+                __syncthreads();
+            }
+            /* (b2) factor and invert the 16 x 16 block */
+            if (wave == 0) {
+                double a[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) a[q] = (q <= li) ? Ld[c0 + li][c0 + q] : 0.0;
+                int fail;
+                block16_potrf(a, lane, fail);
+                if (lane < 16) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) Ld[c0 + li][c0 + q] = (q <= li) ? a[q] : 0.0;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                double col[16];
+                block16_inverse_col(&Ld[c0][c0], OBP, lane, col);
+                if (lane < 16) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) Lb[cb][q][li] = col[q];
+                }
+                if (lane == 0 && fail) s_fail = (C + c0 + fail - 1 < nvalid) ? C + c0 + fail : -1;
+            }
+            __syncthreads();
+            if (s_fail) break;
+            /* (b3) rows [c0+16, W): X = P Lb^T */
+            {
+                const int rb = wave;
+                if (rb < nfb - cb - 1) {
+                    const int r0 = c0 + 16 + 16 * rb;
+                    double pv[4];
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) pv[kk] = Ld[r0 + li][c0 + 4 * kk + lk];
+                    v4d acc = zero4d();
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) acc = mfma_f64(Lb[cb][li][4 * kk + lk], pv[kk], acc);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) Ld[r0 + li][c0 + lk + 4 * s] = acc[s];
+                }
+            }
+            __syncthreads();
+        }
+        if (s_fail) break;
+        /* (c) inverse of the 64 x 64 diagonal block */
+        assemble_inverse(Ld, Lb, Lv, nfb, wave, lane);
+        /* factor back into the tile, inverse into dinv (column-major 64 x 64) */
+        double *dv = dinv + (size_t)(C / OB) * OB * OB;
+        for (int e = tid; e < OB * OB; e += POTRF_THREADS) {
+            const int r = e % OB, c = e / OB;
+            if (r < W && c < W && r >= c) T[(C + r) + (size_t)(C + c) * ldt] = Ld[r][c];
+            dv[e] = Lv[r][c];
+        }
+        /* (d) rows [C+W, ldt): X = P Lv^T, D = Lv P^T */
+        {
+            const int nrb = (ldt - C - W) >> 4;
+            for (int rb = wave; rb < nrb; rb += POTRF_WAVES) {
+                const int r0 = C + W + rb * 16;
+                double pv[16];
+#pragma unroll
+                for (int kq = 0; kq < 16; ++kq)
+                    pv[kq] = (4 * kq < W) ? T[(r0 + li) + (size_t)(C + 4 * kq + lk) * ldt] : 0.0;
+                v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    if (f < nfb) {
+#pragma unroll
+                        for (int kq = 0; kq < 16; ++kq)
+                            if (kq < 4 * (f + 1)) acc[f] = mfma_f64(Lv[16 * f + li][4 * kq + lk], pv[kq], acc[f]);
+                    }
+                }
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    if (f < nfb) {
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) T[(r0 + li) + (size_t)(C + 16 * f + lk + 4 * s) * ldt] = acc[f][s];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && s_fail > 0) *info = iinfo + s_fail;
+}
+
+void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, hipStream_t s)
+{
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(POTRF_THREADS), 0, s, Akk, ldt, dinv, iinfo, nvalid, info);
+}
+
+/* ================================================================== */
+/* 64 x 64 diagonal-block inverses of every diagonal tile of a factor that
+ * was not produced by potrf_diag in this process.  One workgroup (4
+ * wavefronts) per (tile, 64-column block). */
+__global__ __launch_bounds__(256) void dinv64_kernel(const double *A, int nt, long long diag_stride, int ldt,
+                                                      double *dinv)
+{
+    __shared__ double Ld[OB][OBP];
+    __shared__ double Lv[OB][OBP];
+    __shared__ double Lb[4][16][17];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15;
+    const int nob = (ldt + OB - 1) / OB;
+    const int t = blockIdx.x / nob, C = (blockIdx.x % nob) * OB;
+    if (t >= nt) return;
+    const int W = min(OB, ldt - C), nfb = W >> 4;
+    const double *T = A + (size_t)t * diag_stride;
+    for (int e = tid; e < OB * OB; e += 256) {
+        const int r = e % OB, c = e / OB;
+        Ld[r][c] = (r < W && c < W && r >= c) ? T[(C + r) + (size_t)(C + c) * ldt] : 0.0;
+    }
+    __syncthreads();
+    if (wave < nfb) {
+        double col[16];
+        block16_inverse_col(&Ld[16 * wave][16 * wave], OBP, lane, col);
+        if (lane < 16) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) Lb[wave][q][li] = col[q];
+        }
+    }
+    __syncthreads();
+    assemble_inverse(Ld, Lb, Lv, nfb, wave, lane);
+    double *dv = dinv + ((size_t)t * nob + C / OB) * OB * OB;
+    for (int e = tid; e < OB * OB; e += 256) dv[e] = Lv[e % OB][e / OB];
+}
+
+void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s)
+{
+    const int nob = (ldt + OB - 1) / OB;
+    hipLaunchKernelGGL(dinv64_kernel, dim3(nt * nob), dim3(256), 0, s, A, nt, (long long)diag_stride, ldt, dinv);
+}
+
+/* ================================================================== */
+/* Right-side triangular solve X = P L^-T, in place, for a batch of tiles.
+ * One wavefront owns a 16-row strip of one tile and walks its 64-column
+ * blocks left to right:
+ *   R = P[:, C:C+64] - X[:, 0:C] L[C:C+64, 0:C]^T    (MFMA, K = C)
+ *   X[:, C:C+64] = R Linv_C^T                         (MFMA, K = 64)
+ * Tile t uses L0 + t*lstride and the block inverses dinv0 + t*dstride, so
+ * applied to identity tiles the kernel forms Linv^T for every diagonal
+ * tile at once.                                                          */
+__global__ __launch_bounds__(256) void trsm_right_kernel(double *P0, int ntiles, long long pstride, int ldt,
+                                                          const double *L0, long long lstride,
+                                                          const double *dinv0, long long dstride,
+                                                          const int *info)
+{
+    __shared__ double S[4][16][OBP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    if (*info != 0) return;
+    const int nrb = ldt >> 4;
+    const int wid = blockIdx.x * 4 + wave;
+    const int t = wid / nrb, rb = wid % nrb;
+    if (t >= ntiles) return;
+    double *X = P0 + (size_t)t * pstride;
+    const double *L = L0 + (size_t)t * lstride;
+    const double *dinv = dinv0 + (size_t)t * dstride;
+    const int r0 = rb * 16;
+    for (int C = 0; C < ldt; C += OB) {
+        const int W = min(OB, ldt - C), nfb = W >> 4;
+        v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+        wave_gemm_nt_dyn(nfb, X + r0, L + C, ldt, C, lane, acc);
+        for (int f = 0; f < nfb; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = 16 * f + lk + 4 * s;
+                S[wave][li][c] = X[(r0 + li) + (size_t)(C + c) * ldt] - acc[f][s];
+            }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const double *Lv = dinv + (size_t)(C / OB) * OB * OB;   /* column-major 64 x 64 */
+        v4d y[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            if (f < nfb) {
+#pragma unroll
+                for (int kq = 0; kq < 16; ++kq)
+                    if (kq < 4 * (f + 1))
+                        y[f] = mfma_f64(Lv[(16 * f + li) + (size_t)(4 * kq + lk) * OB], S[wave][li][4 * kq + lk], y[f]);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            if (f < nfb) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) X[(r0 + li) + (size_t)(C + 16 * f + lk + 4 * s) * ldt] = y[f][s];
+            }
+        }
+        __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, int ldt,
+                                    const double *L0, long long lstride, const double *dinv0, long long dstride,
+                                    const int *info, hipStream_t s)
+{
+    if (ntiles <= 0) return;
+    const int waves = ntiles * (ldt >> 4);
+    const int blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(trsm_right_kernel, dim3(blocks), dim3(256), 0, s, P0, ntiles, pstride, ldt,
+                       L0, lstride, dinv0, dstride, info);
+}
+
+void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int ldt, const double *Lkk,
+                            const double *dinv, const int *info, hipStream_t s)
+{
+    trsm_right_lower_trans_batched(panel, ntiles, (long long)tile_stride, ldt, Lkk, 0, dinv, 0, info, s);
+}
+
+} /* namespace k */
+} /* namespace chx */

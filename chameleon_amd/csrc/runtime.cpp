@@ -9,6 +9,7 @@
  * synchronises both streams and folds the device info flag into its status.
  */
 #include "internal.h"
+#include "chameleon_amd.h"
 
 #include <cstring>
 #include <cstdarg>
@@ -203,11 +204,41 @@ int request_fail(RUNTIME_sequence_t *seq, RUNTIME_request_t *req, int status)
     return status;
 }
 
+hipEvent_t timing_event()
+{
+    Context *c = g_ctx;
+    if (c->timing_pool.empty()) {
+        hipEvent_t e;
+        CHX_HIP(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = c->timing_pool.back();
+    c->timing_pool.pop_back();
+    return e;
+}
+
+/* fold finished timing pairs into the per-class totals (streams synced) */
+void stats_fold()
+{
+    Context *c = g_ctx;
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        CHX_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        c->st_launches[p.cls] += 1;
+        c->st_ms[p.cls] += ms;
+        c->st_flops[p.cls] += p.flops;
+        c->timing_pool.push_back(p.a);
+        c->timing_pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
 int sequence_wait_internal(RUNTIME_sequence_t *seq)
 {
     Context *c = g_ctx;
     CHX_HIP(hipStreamSynchronize(c->s_update));
     CHX_HIP(hipStreamSynchronize(c->s_main));
+    stats_fold();
     SeqState *st = seq_state(seq);
     if (st && st->pending_info) {
         CHX_HIP(hipMemcpy(st->h_info, st->d_info, sizeof(int), hipMemcpyDeviceToHost));
@@ -280,6 +311,8 @@ int __chameleon_finalize(void)
     }
     CHX_HIP(hipDeviceSynchronize());
     for (hipEvent_t e : c->events) CHX_HIP(hipEventDestroy(e));
+    stats_fold();
+    for (hipEvent_t e : c->timing_pool) CHX_HIP(hipEventDestroy(e));
     if (c->ws) CHX_HIP(hipFree(c->ws));
     if (c->pinned) CHX_HIP(hipHostFree(c->pinned));
     CHX_HIP(hipFree(c->d_info));
@@ -302,7 +335,8 @@ int CHAMELEON_Enable(int option)
     switch (option) {
     case CHAMELEON_WARNINGS: g_ctx->warnings = 1; break;
     case CHAMELEON_AUTOTUNING: g_ctx->autotuning = 1; break;
-    case CHAMELEON_ERRORS: case CHAMELEON_DAG: case CHAMELEON_GENERATE_TRACE: case CHAMELEON_GENERATE_STATS:
+    case CHAMELEON_GENERATE_STATS: g_ctx->stats = true; break;
+    case CHAMELEON_ERRORS: case CHAMELEON_DAG: case CHAMELEON_GENERATE_TRACE:
     case CHAMELEON_PARALLEL_KERNEL: case CHAMELEON_BOUND: case CHAMELEON_PROGRESS: case CHAMELEON_GEMM3M:
     case CHAMELEON_GENERIC:
         break;
@@ -319,7 +353,8 @@ int CHAMELEON_Disable(int option)
     switch (option) {
     case CHAMELEON_WARNINGS: g_ctx->warnings = 0; break;
     case CHAMELEON_AUTOTUNING: g_ctx->autotuning = 0; break;
-    case CHAMELEON_ERRORS: case CHAMELEON_DAG: case CHAMELEON_GENERATE_TRACE: case CHAMELEON_GENERATE_STATS:
+    case CHAMELEON_GENERATE_STATS: g_ctx->stats = false; break;
+    case CHAMELEON_ERRORS: case CHAMELEON_DAG: case CHAMELEON_GENERATE_TRACE:
     case CHAMELEON_PARALLEL_KERNEL: case CHAMELEON_BOUND: case CHAMELEON_PROGRESS: case CHAMELEON_GEMM3M:
     case CHAMELEON_GENERIC:
         break;
@@ -424,6 +459,27 @@ int CHAMELEON_Sequence_Flush(RUNTIME_sequence_t *sequence, RUNTIME_request_t *re
     if (!g_ctx) { error("CHAMELEON_Sequence_Flush", "CHAMELEON not initialized"); return CHAMELEON_ERR_NOT_INITIALIZED; }
     if (!sequence) { error("CHAMELEON_Sequence_Flush", "NULL sequence"); return CHAMELEON_ERR_UNALLOCATED; }
     request_fail(sequence, request, CHAMELEON_ERR_SEQUENCE_FLUSHED);
+    return CHAMELEON_SUCCESS;
+}
+
+int CHAMAMD_Kernel_Stats(int kernel_class, long long *launches, double *total_ms, double *flops)
+{
+    if (!g_ctx) return CHAMELEON_ERR_NOT_INITIALIZED;
+    if (kernel_class < 0 || kernel_class >= CHAMAMD_KERNEL_NCLASS) return CHAMELEON_ERR_ILLEGAL_VALUE;
+    if (launches) *launches = g_ctx->st_launches[kernel_class];
+    if (total_ms) *total_ms = g_ctx->st_ms[kernel_class];
+    if (flops) *flops = g_ctx->st_flops[kernel_class];
+    return CHAMELEON_SUCCESS;
+}
+
+int CHAMAMD_Kernel_Stats_Reset(void)
+{
+    if (!g_ctx) return CHAMELEON_ERR_NOT_INITIALIZED;
+    for (int i = 0; i < CHAMAMD_KERNEL_NCLASS; ++i) {
+        g_ctx->st_launches[i] = 0;
+        g_ctx->st_ms[i] = 0.0;
+        g_ctx->st_flops[i] = 0.0;
+    }
     return CHAMELEON_SUCCESS;
 }
 

@@ -232,6 +232,27 @@ def test_large_tiles_vs_oracle(ch, n, nb, nrhs, uplo):
     assert oracle.check_solve(A0, B, oracle.dplrnt(n, nrhs, 5673))[0]
 
 
+@pytest.mark.parametrize("nb", [40, 80, 100, 144, 200])
+def test_partial_outer_blocks(ch, nb):
+    """Tile sizes whose padded width is not a multiple of 64 exercise the
+    narrower last column block of the POTRF/TRSM kernels."""
+    n, nrhs = 611, 5
+    say("partial outer block", nb)
+    ch.set_param(ch.TILE_SIZE, nb)
+    for uplo in (L_, U_):
+        A = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+        B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+        assert ch.lib.CHAMELEON_dposv(uplo, n, nrhs, _p(A), n, _p(B), n) == 0
+        oracle.use_blas(True)
+        try:
+            Ar, Br, _ = reference_solution(uplo, n, nrhs, nb)
+        finally:
+            oracle.use_blas(False)
+        tri = np.tril if uplo == L_ else np.triu
+        assert oracle.rel_frobenius(tri(A), tri(Ar)) <= TOL_FACTOR
+        assert oracle.rel_frobenius(B, Br) <= TOL_SOLVE
+
+
 # -------------------------------------------------- device-resident workflow
 def test_async_device_resident_matches_sync(ch):
     n, nb, nrhs = 1024, 256, 64
