@@ -45,13 +45,7 @@ static hipEvent_t event(size_t i)
     return c->events[i];
 }
 
-struct PotrfPlan {
-    double *A;
-    int NT, ldt, nb, nvalid_last;   /* nb: descriptor tile size (valid columns of a full tile) */
-    int *d_info;
-};
-
-static void potrf_run(const PotrfPlan &p)
+void potrf_run(const PotrfPlan &p)
 {
     Context *c = ctx();
     hipStream_t sm = c->s_main, su = c->s_update;
@@ -76,8 +70,10 @@ static void potrf_run(const PotrfPlan &p)
     };
     auto update = [&](int k, int n0, int n1, hipStream_t s) {
         if (n0 >= NT) return;
-        timed(CHAMAMD_KERNEL_UPDATE, update_flops(k, n0, n1), s,
-              [&] { k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s); });
+        timed(CHAMAMD_KERNEL_UPDATE, update_flops(k, n0, n1), s, [&] {
+            if (p.lp_mode < 0) k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s);
+            else k::trailing_update_lp(p.A, p.P[k & 1], p.lp_mode, NT, ldt, n0, n1, NT, p.d_info, s);
+        });
     };
     auto panel = [&](int k) {
         const int nvalid = (k == NT - 1) ? p.nvalid_last : p.nb;
@@ -91,6 +87,7 @@ static void potrf_run(const PotrfPlan &p)
             timed(CHAMAMD_KERNEL_TRSM, f, sm, [&] {
                 k::trsm_right_lower_trans(tile(k + 1, k), NT - k - 1, ts, ldt, tile(k, k), dk, p.d_info, sm);
             });
+            if (p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, p.P[k & 1], p.lp_mode, sm);
         }
     };
     if (c->lookahead <= 0 || NT <= 2) {
@@ -121,66 +118,56 @@ static void potrf_run(const PotrfPlan &p)
     CHX_HIP(hipStreamWaitEvent(sm, event(2 * (NT - 2) + 1), 0));
 }
 
-/* Forward (trans = 0: L Y = B) or backward (trans = 1: L^T X = Y) solve of
- * every tile column of B with the Lower factor in A; Tinv holds, for every
- * diagonal tile k, Linv_k^T (upper triangular). */
-static void solve_run(double *A, int NT, int ldt, const double *Tinv, double *B, int Bnt, int wdt, int trans,
-                      int *d_info, int nb, int nvalid_last, int nrhs, int bnb)
-{
-    auto tsz = [&](int k) { return (double)((k == NT - 1) ? nvalid_last : nb); };
-    Context *c = ctx();
-    hipStream_t sm = c->s_main;
-    const size_t ts = (size_t)ldt * ldt;
-    const size_t bts = (size_t)ldt * wdt;
-    /* W (one B tile) lives right after the Tinv tiles in the workspace */
-    double *Wb = const_cast<double *>(Tinv) + (size_t)NT * ts;
-    for (int bn = 0; bn < Bnt; ++bn) {
-        double *Bc = B + (size_t)bn * NT * bts;   /* tile column bn of B: tiles (0..NT-1, bn) */
-        for (int kk = 0; kk < NT; ++kk) {
-            const int k = trans ? NT - 1 - kk : kk;
-            const double *Tk = Tinv + (size_t)k * ts;
-            double *Bk = Bc + (size_t)k * bts;
-            /* W = op(Linv_k) B_k: forward uses Linv = Tk^T, backward Linv^T = Tk */
-            const int cnt = trans ? k : NT - k - 1;
-            const double w = (double)((bn == Bnt - 1) ? nrhs - bn * bnb : bnb);
-            double fl = tsz(k) * tsz(k) * w;
-            for (int m = (trans ? 0 : k + 1); m < (trans ? k : NT); ++m) fl += 2.0 * tsz(m) * tsz(k) * w;
-            timed(CHAMAMD_KERNEL_SOLVE, fl, sm, [&] {
-                k::gemm_batched(trans ? 0 : 1, ldt, wdt, ldt, 1.0, Tk, ldt, 0, Bk, ldt, 0, 0.0, Wb, ldt, 0, 1, nullptr,
-                                d_info, sm);
-                if (!trans) {
-                    /* B_m -= A(m,k) W for m > k, and B_k <- W */
-                    k::gemm_batched(0, ldt, wdt, ldt, -1.0, A + ((size_t)k * NT + k + 1) * ts, ldt, (long long)ts,
-                                    Wb, ldt, 0, 1.0, Bc + (size_t)(k + 1) * bts, ldt, (long long)bts, cnt, Bk, d_info,
-                                    sm);
-                } else {
-                    /* B_m -= A(k,m)^T W for m < k, and B_k <- W */
-                    k::gemm_batched(1, ldt, wdt, ldt, -1.0, A + (size_t)k * ts, ldt, (long long)NT * ts, Wb, ldt, 0,
-                                    1.0, Bc, ldt, (long long)bts, k, Bk, d_info, sm);
-                }
-            });
-        }
-    }
-}
+/* Workspace of the solves: the inverses of every diagonal tile in both
+ * orientations (TT = L_kk^-T, TN = L_kk^-1), the transposed right-hand
+ * sides BT (nrp x NT*ldt), one solved block W and the 64 x 64 inverses. */
 
-/* Tinv_k = Linv_k^T for every diagonal tile, into the workspace.
- * Returns the workspace base (Tinv tiles, then one B-tile-sized W). */
-static double *form_inverses(double *A, int NT, int ldt, int wdt, int *d_info)
+SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info)
 {
     Context *c = ctx();
     hipStream_t sm = c->s_main;
     const size_t ts = (size_t)ldt * ldt;
     const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
-    const size_t dinv_elems = (size_t)NT * dper;
-    const size_t need = ((size_t)NT * ts + (size_t)ldt * wdt + dinv_elems) * sizeof(double);
+    const size_t need = (2 * (size_t)NT * ts + (size_t)nrp * NT * ldt + (size_t)nrp * ldt + (size_t)NT * dper) *
+                        sizeof(double);
     double *ws = (double *)workspace(need);
-    double *Tinv = ws;
-    double *dinv = ws + (size_t)NT * ts + (size_t)ldt * wdt;
-    k::dinv_all(A, NT, (size_t)(NT + 1) * ts, ldt, dinv, sm);
-    k::set_identity(Tinv, NT, ldt, sm);
-    k::trsm_right_lower_trans_batched(Tinv, NT, (long long)ts, ldt, A, (long long)(NT + 1) * ts, dinv,
+    SolvePlan p;
+    p.TT = ws;
+    p.TN = p.TT + (size_t)NT * ts;
+    p.BT = p.TN + (size_t)NT * ts;
+    p.W = p.BT + (size_t)nrp * NT * ldt;
+    p.dinv = p.W + (size_t)nrp * ldt;
+    k::dinv_all(A, NT, (size_t)(NT + 1) * ts, ldt, p.dinv, sm);
+    k::set_identity(p.TT, NT, ldt, sm);
+    k::trsm_right_lower_trans_batched(p.TT, NT, (long long)ts, ldt, A, (long long)(NT + 1) * ts, p.dinv,
                                       (long long)dper, d_info, sm);
-    return Tinv;
+    k::tile_transpose(p.TT, p.TN, NT, ldt, sm);
+    return p;
+}
+
+/* In-place solve of transposed right-hand sides BT (nrp x NT*ldt) with the
+ * Lower factor in A. */
+void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nrp, int *d_info)
+{
+    hipStream_t sm = ctx()->s_main;
+    k::solve_coop(A, p.TN, p.TT, BT, p.W, NT, ldt, nrp, 0, d_info, sm);
+    k::solve_coop(A, p.TN, p.TT, BT, p.W, NT, ldt, nrp, 1, d_info, sm);
+}
+
+/* B <- A^-1 B with the Lower factor in A: forward then backward solve on
+ * the transposed right-hand sides. */
+static void solve_run(const SolvePlan &p, double *A, int NT, int ldt, double *B, int Bnt, int wdt, int *d_info,
+                      double flops)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    const int nrp = Bnt * wdt;
+    timed(CHAMAMD_KERNEL_SOLVE, flops, sm, [&] {
+        k::rhs_transpose(B, p.BT, NT, ldt, wdt, Bnt, 1, sm);
+        k::solve_coop(A, p.TN, p.TT, p.BT, p.W, NT, ldt, nrp, 0, d_info, sm);
+        k::solve_coop(A, p.TN, p.TT, p.BT, p.W, NT, ldt, nrp, 1, d_info, sm);
+        k::rhs_transpose(B, p.BT, NT, ldt, wdt, Bnt, 0, sm);
+    });
 }
 
 static int check_posv_descs(const char *fn, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence,
@@ -260,12 +247,10 @@ int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequen
         error("CHAMELEON_dpotrs", "B tiling does not match A");
         return CHAMELEON_ERR_ILLEGAL_VALUE;
     }
-    double *Tinv = form_inverses((double *)MA->d, MA->mt, MA->ldt, MB->wdt, st->d_info);
-    const int nvl = A->m - (MA->mt - 1) * A->mb;
-    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 0, st->d_info, A->mb, nvl,
-              B->n, B->nb);
-    solve_run((double *)MA->d, MA->mt, MA->ldt, Tinv, (double *)MB->d, MB->nt, MB->wdt, 1, st->d_info, A->mb, nvl,
-              B->n, B->nb);
+    const int nrp = MB->nt * MB->wdt;
+    SolvePlan plan = solve_plan((double *)MA->d, MA->mt, MA->ldt, nrp, st->d_info);
+    solve_run(plan, (double *)MA->d, MA->mt, MA->ldt, (double *)MB->d, MB->nt, MB->wdt, st->d_info,
+              2.0 * (double)A->m * (double)A->m * (double)B->n);
     mirror_mark_written(B);
     B->sync = 1;
     return CHAMELEON_SUCCESS;

@@ -62,6 +62,8 @@ struct Context {
     /* workspace reused across calls, grown on demand */
     void  *ws = nullptr;
     size_t ws_bytes = 0;
+    void  *ws2 = nullptr;
+    size_t ws2_bytes = 0;
     void  *pinned = nullptr;
     size_t pinned_bytes = 0;
     std::vector<hipEvent_t> events;
@@ -95,6 +97,7 @@ inline void timed(int cls, double flops, hipStream_t s, F body)
 }
 
 void *workspace(size_t bytes);        /* context workspace of >= bytes */
+void *workspace2(size_t bytes);       /* second, independent workspace     */
 
 Mirror *mirror_of(const CHAM_desc_t *A);
 /* Make the device mirror of A hold its current values in orientation
@@ -121,6 +124,23 @@ struct SeqState {
 SeqState *seq_state(RUNTIME_sequence_t *seq);
 int sequence_wait_internal(RUNTIME_sequence_t *seq);
 
+/* ---- scheduler pieces shared by the FP64 and mixed-precision paths ---- */
+struct PotrfPlan {
+    double *A;
+    int NT, ldt, nb, nvalid_last;   /* nb: descriptor tile size (valid columns of a full tile) */
+    int *d_info;
+    /* mixed precision: -1 = FP64 trailing updates; 0/1/2 = FP32/BF16/BF16x3
+     * operands, demoted panel alternating between P[0] and P[1] */
+    int lp_mode = -1;
+    void *P[2] = {nullptr, nullptr};
+};
+void potrf_run(const PotrfPlan &p);
+struct SolvePlan {
+    double *TT, *TN, *BT, *W, *dinv;   /* diagonal-tile inverses, transposed RHS, one block, 64x64 inverses */
+};
+SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info);
+void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nrp, int *d_info);
+
 /* ---- kernel launchers (kernels directory) ---- */
 namespace k {
 /* util.hip */
@@ -145,6 +165,23 @@ void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, i
                                     hipStream_t s);
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s);
 void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
+/* mixed.hip */
+void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int mode, hipStream_t s);
+void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, int n0, int n1, int NT, const int *info,
+                        hipStream_t s);
+void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp,
+              hipStream_t s);
+void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStream_t s);
+void axpy(double *Y, const double *X, size_t n, hipStream_t s);
+/* cond.hip */
+void cond_init(double *A, int NT, int ldt, int N, double kappa, hipStream_t s);
+void cond_gemv(const double *A, int NT, int ldt, int N, const double *v, double *w, hipStream_t s);
+void cond_rank2(double *A, int NT, int ldt, int N, const double *v, const double *z, hipStream_t s);
+/* solve.hip */
+void solve_coop(const double *A, const double *TN, const double *TT, double *BT, double *W, int NT, int ldt, int nrhs,
+                int trans, const int *info, hipStream_t s);
+void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s);
+void tile_transpose(const double *src, double *dst, int ntiles, int ldt, hipStream_t s);
 void gemm_batched(int transA, int M, int N, int K, double alpha, const double *A, int lda, long long strideA,
                   const double *B, int ldb, long long strideB, double beta, double *C, int ldc, long long strideC,
                   int batch, double *copy_dst, const int *info, hipStream_t s);

@@ -78,22 +78,35 @@ __device__ __forceinline__ void wave_gemm_nt_dyn(int nf, const double *X, const 
     }
 }
 
+/* 1/sqrt(d) to full double precision: hardware estimate refined by two
+ * Newton-Raphson steps (quadratic convergence from ~2^-22).  NaN and
+ * non-positive d propagate as NaN or are caught by the caller's pivot test. */
+__device__ __forceinline__ double rsqrt_f64(double d)
+{
+    double r = __builtin_amdgcn_rsq(d);
+    r = r * (1.5 - 0.5 * d * r * r);
+    r = r * (1.5 - 0.5 * d * r * r);
+    return r;
+}
+
 /* Cholesky of a 16 x 16 block by one wavefront: lane i (of each 16-lane
  * group; the four groups compute the same thing) holds row i in a[];
  * pivots and column entries travel by lane shuffles.  On return a[] holds
- * row i of L.  `fail` is the 1-based index of the first non-positive pivot
- * (uniform), 0 if none. */
-__device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fail)
+ * row i of L and rd = 1 / L(i,i).  `fail` is the 1-based index of the
+ * first non-positive pivot (uniform), 0 if none. */
+__device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fail, double &rd)
 {
     const int i = lane & 15;
     fail = 0;
+    rd = 0.0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const double d = __shfl(a[j], j, 16);
         if (fail == 0 && !(d > 0.0)) fail = j + 1;
-        const double sd = sqrt(d);
-        const double lij = (i >= j) ? a[j] / sd : 0.0;
-        if (i >= j) a[j] = (i == j) ? sd : lij;
+        const double r = rsqrt_f64(d);
+        const double lij = a[j] * r;                      /* l_ij for i > j, sqrt(d) for i == j */
+        if (i == j) rd = r;
+        if (i >= j) a[j] = lij;
 #pragma unroll
         for (int k = j + 1; k < 16; ++k) {
             const double lkj = __shfl(lij, k, 16);
@@ -103,8 +116,10 @@ __device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fa
 }
 
 /* Column (lane % 16) of the inverse of the lower triangular 16 x 16 block
- * L(r,p) = Ls[r * lds + p], by forward substitution. */
-__device__ __forceinline__ void block16_inverse_col(const double *Ls, int lds, int lane, double (&col)[16])
+ * L(r,p) = Ls[r * lds + p], by forward substitution, with the reciprocal
+ * diagonal rdiag[r] = 1 / L(r,r). */
+__device__ __forceinline__ void block16_inverse_col(const double *Ls, int lds, const double *rdiag, int lane,
+                                                    double (&col)[16])
 {
     const int jc = lane & 15;
 #pragma unroll
@@ -112,7 +127,7 @@ __device__ __forceinline__ void block16_inverse_col(const double *Ls, int lds, i
         double s = (r == jc) ? 1.0 : 0.0;
 #pragma unroll
         for (int p = 0; p < r; ++p) s -= Ls[r * lds + p] * col[p];
-        col[r] = (r < jc) ? 0.0 : s / Ls[r * lds + r];
+        col[r] = (r < jc) ? 0.0 : s * rdiag[r];
     }
 }
 
@@ -170,28 +185,74 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(double *T, in
     __shared__ double Ld[OB][OBP];
     __shared__ double Lv[OB][OBP];
     __shared__ double Lb[4][16][17];
+    __shared__ double Rd[16];
+    __shared__ double Ys[2][16 * OB];   /* Y slice of phase (a): 64 rows x 16 k, staged per slice */
     __shared__ int s_fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     if (*info != 0) return;
+    /* this kernel is on the critical path of the factorisation: its waves
+     * win issue arbitration against co-resident trailing-update waves */
+    __builtin_amdgcn_s_setprio(2);
     if (tid == 0) s_fail = 0;
     __syncthreads();
 
     for (int C = 0; C < ldt; C += OB) {
         const int W = min(OB, ldt - C), nfb = W >> 4;
-        /* (a) */
+        /* (a) rows [C, ldt) x cols [C, C+W) -= T[., 0:C] T[C:C+W, 0:C]^T.
+         * The 16-deep slices of the W shared rows (Y) are staged in LDS,
+         * double-buffered, so one CU's L2 bandwidth carries each slice
+         * once; every wavefront streams its own 16 rows (X). */
         if (C > 0) {
             const int nrb = (ldt - C) >> 4;
-            for (int rb = wave; rb < nrb; rb += POTRF_WAVES) {
+            for (int rb0 = 0; rb0 < nrb; rb0 += POTRF_WAVES) {
+                const int rb = rb0 + wave;
+                const bool active = rb < nrb;
                 const int r0 = C + rb * 16;
                 v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
-                wave_gemm_nt_dyn(nfb, T + r0, T + C, ldt, C, lane, acc);
-                for (int f = 0; f < nfb; ++f)
+                const double *px = T + (r0 + li) + (size_t)lk * ldt;
+                /* Y slice element (row r < 64, k q < 16) -> Ys[q * 64 + r];
+                 * thread tid moves rows 2*(tid%32)..+1 of column tid/32 */
+                const int yq = tid >> 5, yr = (tid & 31) * 2;
+                const bool yv = yr < W;
+                v2d yreg = {0.0, 0.0};
+                if (yv) yreg = *reinterpret_cast<const v2d *>(T + (C + yr) + (size_t)yq * ldt);
+                double x[4];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const int r = r0 + li, c = C + 16 * f + lk + 4 * s;
-                        if (r >= c) T[r + (size_t)c * ldt] -= acc[f][s];
+                for (int q = 0; q < 4; ++q) x[q] = active ? px[(size_t)(4 * q) * ldt] : 0.0;
+                *reinterpret_cast<v2d *>(&Ys[0][yq * OB + yr]) = yreg;
+                __syncthreads();
+                for (int p = 0; p < C; p += 16) {
+                    const int cur = (p >> 4) & 1;
+                    const bool more = p + 16 < C;
+                    double xn[4];
+                    if (more) {
+                        if (yv) yreg = *reinterpret_cast<const v2d *>(T + (C + yr) + (size_t)(p + 16 + yq) * ldt);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) xn[q] = active ? px[(size_t)(p + 16 + 4 * q) * ldt] : 0.0;
                     }
+                    if (active) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int f = 0; f < 4; ++f)
+                                if (f < nfb) acc[f] = mfma_f64(Ys[cur][(4 * q + lk) * OB + 16 * f + li], x[q], acc[f]);
+                    }
+                    if (more) {
+                        *reinterpret_cast<v2d *>(&Ys[cur ^ 1][yq * OB + yr]) = yreg;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) x[q] = xn[q];
+                    }
+                    __syncthreads();
+                }
+                if (active) {
+                    for (int f = 0; f < nfb; ++f)
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            const int r = r0 + li, c = C + 16 * f + lk + 4 * s;
+                            if (r >= c) T[r + (size_t)c * ldt] -= acc[f][s];
+                        }
+                }
             }
             __syncthreads();
         }
@@ -225,16 +286,18 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(double *T, in
 #pragma unroll
                 for (int q = 0; q < 16; ++q) a[q] = (q <= li) ? Ld[c0 + li][c0 + q] : 0.0;
                 int fail;
-                block16_potrf(a, lane, fail);
+                double rd;
+                block16_potrf(a, lane, fail, rd);
                 if (lane < 16) {
 #pragma unroll
                     for (int q = 0; q < 16; ++q) Ld[c0 + li][c0 + q] = (q <= li) ? a[q] : 0.0;
+                    Rd[li] = rd;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 double col[16];
-                block16_inverse_col(&Ld[c0][c0], OBP, lane, col);
+                block16_inverse_col(&Ld[c0][c0], OBP, Rd, lane, col);
                 if (lane < 16) {
 #pragma unroll
                     for (int q = 0; q < 16; ++q) Lb[cb][q][li] = col[q];
@@ -330,8 +393,10 @@ __global__ __launch_bounds__(256) void dinv64_kernel(const double *A, int nt, lo
     }
     __syncthreads();
     if (wave < nfb) {
-        double col[16];
-        block16_inverse_col(&Ld[16 * wave][16 * wave], OBP, lane, col);
+        double col[16], rdiag[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) rdiag[r] = 1.0 / Ld[16 * wave + r][16 * wave + r];
+        block16_inverse_col(&Ld[16 * wave][16 * wave], OBP, rdiag, lane, col);
         if (lane < 16) {
 #pragma unroll
             for (int q = 0; q < 16; ++q) Lb[wave][q][li] = col[q];
@@ -367,6 +432,7 @@ __global__ __launch_bounds__(256) void trsm_right_kernel(double *P0, int ntiles,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     if (*info != 0) return;
+    __builtin_amdgcn_s_setprio(2);   /* critical path, see potrf_diag_kernel */
     const int nrb = ldt >> 4;
     const int wid = blockIdx.x * 4 + wave;
     const int t = wid / nrb, rb = wid % nrb;

@@ -1,0 +1,305 @@
+/*
+ * mixed.cpp -- mixed-precision Cholesky solve with FP64 iterative
+ * refinement, and the FP64 backward-error report.
+ *
+ * The semantics are LAPACK dsposv's (the routine Chameleon declares as
+ * CHAMELEON_zcposv / dsposv but does not build, include/chameleon/
+ * chameleon_zc.h:34,42):
+ *   - A is factored in reduced precision; X0 = A_lp^-1 B;
+ *   - R = B - A X in FP64; column j has converged when
+ *       ||R_j||_inf <= ||X_j||_inf * ||A||_inf * eps * sqrt(N);
+ *   - otherwise X += A_lp^-1 R, at most ITERMAX = 30 times;
+ *   - on a reduced-precision factorisation failure (ITER = -3) or no
+ *     convergence (ITER = -31) the plain FP64 factorisation is used and A
+ *     holds its factor on return; otherwise A is unchanged.
+ * ITER >= 0 is the number of refinement steps taken.
+ *
+ * Reduced precision here means reduced-precision *products*: the working
+ * copy of A stays in FP64 storage, its panels are factored in FP64, and the
+ * trailing updates multiply FP32, BF16 or split-BF16 operands on the
+ * matching MFMA units with FP32 accumulation (kernels/mixed.hip).
+ */
+#include "internal.h"
+#include "chameleon_amd.h"
+#include "chameleon/chameleon_ds.h"
+
+#include <cmath>
+#include <vector>
+
+using namespace chx;
+
+namespace chx {
+int potrf_async(cham_uplo_t uplo, CHAM_desc_t *A, RUNTIME_sequence_t *sequence);
+int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence);
+}
+
+namespace {
+
+const int ITERMAX = 30;
+
+double norm_inf_sym(CHAM_desc_t *A)
+{
+    /* the mirror is in Lower orientation (possibly transposed storage of an
+     * Upper matrix): the symmetric inf-norm of its lower triangle */
+    Context *c = ctx();
+    Mirror *M = mirror_of(A);
+    const size_t n = (size_t)A->m + A->n + 2;
+    double *work = (double *)workspace(n * sizeof(double));
+    CHX_HIP(hipMemsetAsync(work, 0, n * sizeof(double), c->s_main));
+    k::norm_partials((const double *)M->d, M->mt, M->nt, M->ldt, M->wdt, A->mb, A->nb, A->m, A->n, ChamLower, 0,
+                     work, work + A->n, work + A->n + A->m, c->s_main);
+    std::vector<double> h(A->m);
+    CHX_HIP(hipMemcpyAsync(h.data(), work + A->n, A->m * sizeof(double), hipMemcpyDeviceToHost, c->s_main));
+    CHX_HIP(hipStreamSynchronize(c->s_main));
+    double r = 0.0;
+    for (double v : h) r = std::fmax(r, v);
+    return r;
+}
+
+/* column max-norms of a transposed block (nrp x rows) */
+std::vector<double> colmax_host(const double *V, int nrp, int nvalid, int rows, double *dwork)
+{
+    Context *c = ctx();
+    CHX_HIP(hipMemsetAsync(dwork, 0, nrp * sizeof(double), c->s_main));
+    k::colmax(V, nrp, nvalid, rows, dwork, c->s_main);
+    std::vector<double> h(nvalid);
+    CHX_HIP(hipMemcpyAsync(h.data(), dwork, nvalid * sizeof(double), hipMemcpyDeviceToHost, c->s_main));
+    CHX_HIP(hipStreamSynchronize(c->s_main));
+    return h;
+}
+
+int check_descs(const char *fn, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X)
+{
+    if (!ctx()) {
+        error(fn, "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    CHAM_desc_t *d[3] = {A, B, X};
+    for (int i = 0; i < 3; ++i) {
+        if (desc_check(d[i]) != CHAMELEON_SUCCESS || cham_get_flttype(d[i]->dtyp) != ChamRealDouble) {
+            error(fn, "invalid descriptor");
+            return CHAMELEON_ERR_ILLEGAL_VALUE;
+        }
+        if (d[i]->mb != d[i]->nb) {
+            error(fn, "only square tiles supported");
+            return CHAMELEON_ERR_ILLEGAL_VALUE;
+        }
+    }
+    if (A->m != A->n || B->m != A->m || X->m != A->m || B->n != X->n || B->mb != A->mb || X->mb != A->mb) {
+        error(fn, "inconsistent dimensions");
+        return CHAMELEON_ERR_ILLEGAL_VALUE;
+    }
+    return CHAMELEON_SUCCESS;
+}
+
+} /* namespace */
+
+namespace chx {
+
+/* Backward error max_j ||B_j - A X_j||_inf / (||A||_inf ||X_j||_inf N eps)
+ * on the device, A symmetric given by its `uplo` triangle. */
+int backward_error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t *B, double *berr)
+{
+    Context *c = ctx();
+    hipStream_t s = c->s_main;
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, s);
+    mirror_acquire(X, MIRROR_NORMAL, false, s);
+    mirror_acquire(B, MIRROR_NORMAL, false, s);
+    Mirror *MA = mirror_of(A), *MX = mirror_of(X), *MB = mirror_of(B);
+    const int NT = MA->mt, ldt = MA->ldt, nrp = MX->nt * MX->wdt, rows = NT * ldt;
+    const double anrm = norm_inf_sym(A);
+    const size_t vb = (size_t)nrp * rows;
+    double *w = (double *)workspace2((3 * vb + nrp) * sizeof(double));
+    double *XT = w, *BT = w + vb, *RT = w + 2 * vb, *nw = w + 3 * vb;
+    k::rhs_transpose((double *)MX->d, XT, NT, ldt, MX->wdt, MX->nt, 1, s);
+    k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
+    k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, s);
+    std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
+    std::vector<double> rn = colmax_host(RT, nrp, nrp, rows, nw);
+    const double eps = std::ldexp(1.0, -53);
+    double worst = 0.0;
+    for (int j = 0; j < X->n; ++j) {
+        const int pj = (j / X->nb) * MX->wdt + (j % X->nb);   /* padded column index */
+        const double d = anrm * xn[pj] * A->m * eps;
+        worst = std::fmax(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
+    }
+    *berr = worst;
+    return CHAMELEON_SUCCESS;
+}
+
+int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec, int *iter, double *berr)
+{
+    Context *c = ctx();
+    hipStream_t s = c->s_main;
+    const int orient = uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL;
+    mirror_acquire(A, orient, true, s);
+    mirror_acquire(B, MIRROR_NORMAL, false, s);
+    mirror_prepare_overwrite(X, MIRROR_NORMAL, false);
+    Mirror *MA = mirror_of(A), *MB = mirror_of(B), *MX = mirror_of(X);
+    const int NT = MA->mt, ldt = MA->ldt, N = A->m;
+    const int nrp = MB->nt * MB->wdt, rows = NT * ldt;
+    const size_t ts = (size_t)ldt * ldt;
+    const double eps = std::ldexp(1.0, -53);
+    int it_count = 0;
+    double worst = 0.0;
+    bool converged = false, lp_ok = (ldt % 128 == 0);
+
+    RUNTIME_sequence_t *seq = nullptr;
+    CHAMELEON_Sequence_Create(&seq);
+    SeqState *st = seq_state(seq);
+
+    if (lp_ok) {
+        const double anrm = norm_inf_sym(A);
+        const double cte = anrm * eps * std::sqrt((double)N);
+        /* workspace2: working factor | panels (2) | BT | XT | RT | norms */
+        const size_t abytes = MA->bytes;
+        const size_t pbytes = (size_t)NT * ts * 4;
+        const size_t vb = (size_t)nrp * rows;
+        char *w = (char *)workspace2(abytes + 2 * pbytes + (3 * vb + nrp) * sizeof(double));
+        double *Aw = (double *)w;
+        void *P0 = w + abytes, *P1 = w + abytes + pbytes;
+        double *BT = (double *)(w + abytes + 2 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb;
+        CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
+
+        PotrfPlan p;
+        p.A = Aw;
+        p.NT = NT;
+        p.ldt = ldt;
+        p.nb = A->nb;
+        p.nvalid_last = N - (NT - 1) * A->mb;
+        p.d_info = st->d_info;
+        p.lp_mode = prec;
+        p.P[0] = P0;
+        p.P[1] = P1;
+        potrf_run(p);
+        st->pending_info = true;
+        const int info = sequence_wait_internal(seq);
+        if (info != 0) {
+            it_count = -3;
+        } else {
+            SolvePlan sp = solve_plan(Aw, NT, ldt, nrp, st->d_info);
+            k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
+            CHX_HIP(hipMemcpyAsync(XT, BT, vb * sizeof(double), hipMemcpyDeviceToDevice, s));
+            solve_bt(sp, Aw, NT, ldt, XT, nrp, st->d_info);
+            for (int it = 0; it <= ITERMAX; ++it) {
+                k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, s);
+                std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
+                std::vector<double> rn = colmax_host(RT, nrp, nrp, rows, nw);
+                bool ok = true;
+                worst = 0.0;
+                for (int j = 0; j < X->n; ++j) {
+                    const int pj = (j / X->nb) * MX->wdt + (j % X->nb);
+                    if (!(rn[pj] <= xn[pj] * cte)) ok = false;
+                    const double d = anrm * xn[pj] * N * eps;
+                    worst = std::fmax(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
+                }
+                if (ok) {
+                    converged = true;
+                    it_count = it;
+                    break;
+                }
+                if (it == ITERMAX) break;
+                solve_bt(sp, Aw, NT, ldt, RT, nrp, st->d_info);
+                k::axpy(XT, RT, vb, s);
+            }
+            if (converged) {
+                k::rhs_transpose((double *)MX->d, XT, NT, ldt, MX->wdt, MX->nt, 0, s);
+                CHX_HIP(hipStreamSynchronize(s));
+            } else {
+                it_count = -(ITERMAX + 1);
+            }
+        }
+    } else {
+        it_count = -1;
+    }
+    int status = CHAMELEON_SUCCESS;
+    if (!converged) {
+        /* plain FP64 factorisation and solve; A holds its factor */
+        CHAMELEON_Sequence_Destroy(seq);
+        CHAMELEON_Sequence_Create(&seq);
+        CHX_HIP(hipMemcpyAsync(MX->d, MB->d, MB->bytes, hipMemcpyDeviceToDevice, s));
+        MX->state = MB->state;
+        MX->diag_identity = false;
+        potrf_async(uplo, A, seq);
+        potrs_async(uplo, A, X, seq);
+        status = sequence_wait_internal(seq);
+        worst = -1.0;
+    }
+    CHAMELEON_Sequence_Destroy(seq);
+    mirror_mark_written(X);
+    X->sync = 1;
+    if (iter) *iter = it_count;
+    if (berr) *berr = worst;
+    return status;
+}
+
+} /* namespace chx */
+
+extern "C" {
+
+int CHAMAMD_dposv_mixed_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec, int *iter,
+                             double *berr)
+{
+    int rc = check_descs("CHAMAMD_dposv_mixed_Tile", A, B, X);
+    if (rc != CHAMELEON_SUCCESS) return rc;
+    if (uplo != ChamUpper && uplo != ChamLower) {
+        error("CHAMAMD_dposv_mixed_Tile", "illegal value of uplo");
+        return -1;
+    }
+    if (prec < CHAMAMD_PREC_FP32 || prec > CHAMAMD_PREC_BF16X3) {
+        error("CHAMAMD_dposv_mixed_Tile", "illegal value of prec");
+        return -5;
+    }
+    if (A->m == 0 || B->n == 0) {
+        if (iter) *iter = 0;
+        return CHAMELEON_SUCCESS;
+    }
+    rc = mixed_posv(uplo, A, B, X, prec, iter, berr);
+    CHAMELEON_Desc_Flush(X, nullptr);
+    CHAMELEON_Desc_Flush(A, nullptr);
+    CHX_HIP(hipStreamSynchronize(ctx()->s_main));
+    return rc;
+}
+
+int CHAMAMD_Backward_Error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t *B, double *berr)
+{
+    int rc = check_descs("CHAMAMD_Backward_Error", A, B, X);
+    if (rc != CHAMELEON_SUCCESS) return rc;
+    return backward_error(uplo, A, X, B, berr);
+}
+
+int CHAMELEON_dsposv_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int *ITER)
+{
+    return CHAMAMD_dposv_mixed_Tile(uplo, A, B, X, CHAMAMD_PREC_FP32, ITER, nullptr);
+}
+
+int CHAMELEON_dsposv(cham_uplo_t uplo, int N, int NRHS, double *A, int LDA, double *B, int LDB, double *X, int LDX,
+                     int *ITER)
+{
+    if (!ctx()) {
+        error("CHAMELEON_dsposv", "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    if (uplo != ChamUpper && uplo != ChamLower) { error("CHAMELEON_dsposv", "illegal value of uplo"); return -1; }
+    if (N < 0) { error("CHAMELEON_dsposv", "illegal value of N"); return -2; }
+    if (NRHS < 0) { error("CHAMELEON_dsposv", "illegal value of NRHS"); return -3; }
+    if (LDA < (N > 1 ? N : 1)) { error("CHAMELEON_dsposv", "illegal value of LDA"); return -5; }
+    if (LDB < (N > 1 ? N : 1)) { error("CHAMELEON_dsposv", "illegal value of LDB"); return -7; }
+    if (LDX < (N > 1 ? N : 1)) { error("CHAMELEON_dsposv", "illegal value of LDX"); return -9; }
+    if (N == 0 || NRHS == 0) {
+        if (ITER) *ITER = 0;
+        return CHAMELEON_SUCCESS;
+    }
+    const int nb = ctx()->nb;
+    CHAM_desc_t *dA = nullptr, *dB = nullptr, *dX = nullptr;
+    CHAMELEON_Desc_Create(&dA, A, ChamRealDouble, nb, nb, nb * nb, LDA, N, 0, 0, N, N, 1, 1);
+    CHAMELEON_Desc_Create(&dB, B, ChamRealDouble, nb, nb, nb * nb, LDB, NRHS, 0, 0, N, NRHS, 1, 1);
+    CHAMELEON_Desc_Create(&dX, X, ChamRealDouble, nb, nb, nb * nb, LDX, NRHS, 0, 0, N, NRHS, 1, 1);
+    int rc = (dA && dB && dX) ? CHAMELEON_dsposv_Tile(uplo, dA, dB, dX, ITER) : CHAMELEON_ERR_ILLEGAL_VALUE;
+    if (dA) CHAMELEON_Desc_Destroy(&dA);
+    if (dB) CHAMELEON_Desc_Destroy(&dB);
+    if (dX) CHAMELEON_Desc_Destroy(&dX);
+    return rc;
+}
+
+} /* extern "C" */

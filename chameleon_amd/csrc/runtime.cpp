@@ -52,6 +52,19 @@ void *workspace(size_t bytes)
     return c->ws;
 }
 
+void *workspace2(size_t bytes)
+{
+    Context *c = g_ctx;
+    if (bytes <= c->ws2_bytes) return c->ws2;
+    if (c->ws2) {
+        CHX_HIP(hipDeviceSynchronize());
+        CHX_HIP(hipFree(c->ws2));
+    }
+    CHX_HIP(hipMalloc(&c->ws2, bytes));
+    c->ws2_bytes = bytes;
+    return c->ws2;
+}
+
 /* ------------------------------------------------------------ mirrors */
 static int rup16(int x) { return (x + 15) & ~15; }
 
@@ -314,6 +327,7 @@ int __chameleon_finalize(void)
     stats_fold();
     for (hipEvent_t e : c->timing_pool) CHX_HIP(hipEventDestroy(e));
     if (c->ws) CHX_HIP(hipFree(c->ws));
+    if (c->ws2) CHX_HIP(hipFree(c->ws2));
     if (c->pinned) CHX_HIP(hipHostFree(c->pinned));
     CHX_HIP(hipFree(c->d_info));
     CHX_HIP(hipStreamDestroy(c->s_main));

@@ -30,6 +30,34 @@ enum {
 int CHAMAMD_Kernel_Stats(int kernel_class, long long *launches, double *total_ms, double *flops);
 int CHAMAMD_Kernel_Stats_Reset(void);
 
+/* Products used by the trailing updates of the mixed-precision solve (the
+ * matrix itself stays in FP64 storage, panels are factored in FP64). */
+enum {
+    CHAMAMD_PREC_FP32   = 0,   /* FP32 operands, v_mfma_f32_16x16x4_f32                      */
+    CHAMAMD_PREC_BF16   = 1,   /* BF16 operands, v_mfma_f32_16x16x32_bf16                    */
+    CHAMAMD_PREC_BF16X3 = 2    /* operands split hi+lo in BF16, 3 products (~2^-16 accuracy) */
+};
+
+/* Mixed-precision Cholesky solve X = A^-1 B with FP64 iterative refinement
+ * and FP64 fallback (LAPACK dsposv semantics, see chameleon/chameleon_ds.h).
+ * `iter` receives the ITER code, `berr` the final backward error
+ * max_j ||B_j - A X_j||_inf / (||A||_inf ||X_j||_inf N eps) (-1 when the
+ * FP64 fallback was taken).  B is not modified. */
+int CHAMAMD_dposv_mixed_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec,
+                             int *iter, double *berr);
+
+/* Symmetric positive definite matrix with eigenvalues kappa^(-i/(N-1)),
+ * i < N, generated on the device: A = Q diag(lambda) Q^T with Q the
+ * product of `nrefl` random Householder reflectors (seeded).  Conditioned
+ * test matrices in the role of the reference's dlatms generator
+ * (compute/pzlatms.c, mode 3 spectrum of core_zlatm1.c), at O(nrefl N^2)
+ * cost.  The full matrix (both triangles) is written. */
+int CHAMAMD_dplgsy_cond_Tile(double kappa, int nrefl, CHAM_desc_t *A, unsigned long long int seed);
+
+/* The same backward error for any X, A symmetric given by its `uplo`
+ * triangle, computed on the device in FP64. */
+int CHAMAMD_Backward_Error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t *B, double *berr);
+
 #ifdef __cplusplus
 }
 #endif

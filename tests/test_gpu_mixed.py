@@ -1,0 +1,168 @@
+"""GPU tests of the mixed-precision solve with FP64 iterative refinement
+(CHAMELEON_dsposv / CHAMAMD_dposv_mixed_Tile) and of the FP64 backward
+error report.
+
+Parity bar: whatever the product precision, a converged refinement must
+give the FP64 answer -- the solution within cond(A) * 1e-13 of an FP64
+reference solve and a scaled residual under Chameleon's threshold (60);
+a refinement that cannot converge must fall back to FP64 (ITER < 0) with
+the same guarantees.  A is unchanged when refinement succeeded.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+L_, U_, UL_ = 122, 121, 123
+FP32, BF16, BF16X3 = 0, 1, 2
+
+
+@pytest.fixture(scope="module")
+def ch():
+    import chameleon_amd as ch
+    ch.init()
+    L = ch.load()
+    L.CHAMELEON_dsposv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int)]
+    L.CHAMAMD_dposv_mixed_Tile.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    L.CHAMAMD_Backward_Error.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_double)]
+    yield ch
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def conditioned_spd(n, kappa, seed):
+    """A = Q diag(lambda) Q^T, Q from QR of a seeded Gaussian matrix,
+    lambda geometric from 1 to 1/kappa (the construction of
+    compute/pzlatms.c mode 3, core_zlatm1.c:172-180)."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = kappa ** (-np.arange(n) / (n - 1))
+    A = (Q * lam) @ Q.T
+    return np.asfortranarray((A + A.T) / 2)
+
+
+def mixed(ch, uplo, A, B, prec, nb):
+    n, nrhs = B.shape
+    X = np.zeros_like(B, order="F")
+    it, be = ctypes.c_int(-99), ctypes.c_double(-2.0)
+    with ch.Desc.from_array(A, nb) as dA, ch.Desc.from_array(B, nb) as dB, ch.Desc.from_array(X, nb) as dX:
+        rc = ch.lib.CHAMAMD_dposv_mixed_Tile(uplo, dA.ptr, dB.ptr, dX.ptr, prec, ctypes.byref(it), ctypes.byref(be))
+    return rc, X, it.value, be.value
+
+
+@pytest.mark.parametrize("prec", [FP32, BF16X3, BF16])
+@pytest.mark.parametrize("uplo", [L_, U_])
+@pytest.mark.parametrize("n,nb,nrhs", [(1024, 256, 1), (1536, 256, 16), (2048, 512, 64), (1000, 128, 3)])
+def test_mixed_well_conditioned(ch, prec, uplo, n, nb, nrhs):
+    say("mixed bumped", prec, uplo, n, nb, nrhs)
+    A = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+    A_in = A.copy(order="F")
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    rc, X, it, be = mixed(ch, uplo, A, B, prec, nb)
+    assert rc == 0
+    assert 0 <= it <= 30, it
+    assert np.array_equal(A, A_in)                     # A unchanged after refinement
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    Xr = np.linalg.solve(A0, B)
+    assert oracle.rel_frobenius(X, Xr) <= 1e-13
+    assert oracle.check_solve(A0, X, B)[0]
+    assert 0.0 <= be <= 1.0
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+@pytest.mark.parametrize("prec", [FP32, BF16X3, BF16])
+def test_mixed_conditioned(ch, kappa, prec):
+    n, nb, nrhs = 1024, 256, 4
+    say("mixed conditioned", kappa, prec)
+    A = conditioned_spd(n, kappa, 7)
+    A0 = A.copy(order="F")
+    rng = np.random.default_rng(3)
+    B = np.asfortranarray(rng.uniform(-0.5, 0.5, (n, nrhs)))
+    rc, X, it, be = mixed(ch, L_, A, B, prec, nb)
+    assert rc == 0
+    Xr = np.linalg.solve(A0, B)
+    # forward error bounded by cond(A) times a modest multiple of eps
+    assert oracle.rel_frobenius(X, Xr) <= kappa * 1e-13
+    assert oracle.check_solve(A0, X, B)[0]
+    if it >= 0:
+        assert np.array_equal(A, A0)
+    else:
+        assert it in (-3, -31)
+    if prec == FP32:
+        assert it >= 0      # kappa * 2^-24 << 1: FP32 products always converge here
+
+
+def test_dsposv_lapack_layout(ch):
+    n, nrhs, nb = 1200, 5, 256
+    say("dsposv lapack")
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), L_, n, 51, lda=n + 3, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, lda=n + 5, nb=nb)
+    X = np.zeros((n + 7, nrhs), order="F")
+    it = ctypes.c_int(-99)
+    rc = ch.lib.CHAMELEON_dsposv(L_, n, nrhs, _p(A), n + 3, _p(B), n + 5, _p(X), n + 7, ctypes.byref(it))
+    assert rc == 0 and it.value >= 0
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    assert oracle.rel_frobenius(X[:n], np.linalg.solve(A0, B[:n])) <= 1e-13
+    assert ch.lib.CHAMELEON_dsposv(L_, n, nrhs, _p(A), n + 3, _p(B), n + 5, _p(X), n - 1, ctypes.byref(it)) == -9
+
+
+def test_mixed_untileable_uses_fp64(ch):
+    n, nb, nrhs = 500, 100, 2        # padded tile 112: not a multiple of 128
+    say("mixed fallback -1")
+    A = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    rc, X, it, be = mixed(ch, L_, A, B, FP32, nb)
+    assert rc == 0 and it == -1
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    assert oracle.rel_frobenius(X, np.linalg.solve(A0, B)) <= 1e-12
+    F = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+    assert oracle.dpotrf(L_, F, n, nb) == 0
+    assert oracle.rel_frobenius(np.tril(A), np.tril(F)) <= 1e-12   # A holds the FP64 factor
+
+
+def test_mixed_non_spd_reports_info(ch, golden):
+    n, nb = 256, 128
+    say("mixed non-spd")
+    A = oracle.dplgsy(0.0, UL_, n, 3)
+    _, info_ref = __import__("scipy.linalg.lapack", fromlist=["dpotrf"]).dpotrf(A.copy(), lower=True)
+    B = np.ones((n, 1), order="F")
+    rc, X, it, be = mixed(ch, L_, A, B, FP32, nb)
+    assert it == -3
+    assert rc == info_ref > 0
+
+
+def test_backward_error_of_fp64_solution(ch):
+    n, nb, nrhs = 2048, 256, 8
+    say("backward error")
+    A = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    X = B.copy(order="F")
+    ch.set_param(ch.TILE_SIZE, nb)
+    assert ch.lib.CHAMELEON_dposv(L_, n, nrhs, _p(A), n, _p(X), n) == 0
+    A0 = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+    be = ctypes.c_double(-1)
+    with ch.Desc.from_array(A0, nb) as dA, ch.Desc.from_array(X, nb) as dX, ch.Desc.from_array(B, nb) as dB:
+        assert ch.lib.CHAMAMD_Backward_Error(L_, dA.ptr, dX.ptr, dB.ptr, ctypes.byref(be)) == 0
+    Af = oracle.dplgsy(float(n), UL_, n, 51)
+    R = B - Af @ X
+    ref = max(np.abs(R[:, j]).max() / (np.abs(Af).sum(1).max() * np.abs(X[:, j]).max() * n * 2.0 ** -53)
+              for j in range(nrhs))
+    # both are rounding-level quantities summed in different orders
+    assert 0.0 < be.value < 1.0 and 0.25 <= be.value / ref <= 4.0
