@@ -16,6 +16,15 @@ Reported (one JSON line):
                with OpenBLAS tile kernels) on the host cores, on the
                reference's CPU configuration (N=4096, NB=320, NRHS=16)
   latency_ms   p50 / p99 of the per-step dposv times
+  solve_latency_ms  p50 / p99 of dpotrs with the factor resident in HBM
+               (NRHS = 1 and 64), the per-request cost of a repeated solve
+  mixed        for SPD matrices of condition number kappa (1e2, 1e4, 1e6;
+               eigenvalues kappa^(-i/(N-1)), random orthogonal eigenvectors,
+               generated on the device): the FP64 dposv time and backward
+               error next to the mixed-precision solve with FP64 iterative
+               refinement (FP32, split-BF16 and BF16 trailing products) --
+               time, speed-up over FP64, refinement steps and the final FP64
+               backward error ||b - A x||_inf / (||A||_inf ||x||_inf N eps)
 """
 import argparse
 import ctypes
@@ -82,6 +91,128 @@ def gpu_dposv(ch, n, nrhs, nb, steps, warmup, stats=True):
     return times, kstats
 
 
+def solve_latency(ch, n, nb, reps):
+    """dpotrs latency with the factor already resident on the device."""
+    L = ch.lib
+    ch.set_param(ch.TILE_SIZE, nb)
+    out = {}
+    with ch.Desc.alloc(n, n, nb) as dA:
+        seq = ch.Sequence()
+        ch.check(L.CHAMELEON_dplgsy_Tile_Async(float(n), 123, dA.ptr, 51, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dpotrf_Tile_Async(ch.LOWER, dA.ptr, seq.ptr, seq.req))
+        ch.check(seq.wait())
+        seq.destroy()
+        for nrhs in (1, 64):
+            lat = []
+            with ch.Desc.alloc(n, nrhs, nb) as dB:
+                for it in range(reps + 2):
+                    seq = ch.Sequence()
+                    ch.check(L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673 + it, seq.ptr, seq.req))
+                    ch.check(seq.wait())
+                    seq.destroy()
+                    seq = ch.Sequence()
+                    t0 = time.perf_counter()
+                    ch.check(L.CHAMELEON_dpotrs_Tile_Async(ch.LOWER, dA.ptr, dB.ptr, seq.ptr, seq.req))
+                    ch.check(seq.wait())
+                    t1 = time.perf_counter()
+                    seq.destroy()
+                    if it >= 2:
+                        lat.append((t1 - t0) * 1e3)
+            lat = np.array(lat)
+            out[f"nrhs{nrhs}"] = {"p50": round(float(np.percentile(lat, 50)), 3),
+                                  "p99": round(float(np.percentile(lat, 99)), 3), "calls": len(lat)}
+            print(f"[bench] dpotrs n={n} nrhs={nrhs}: p50 {out[f'nrhs{nrhs}']['p50']} ms "
+                  f"p99 {out[f'nrhs{nrhs}']['p99']} ms", file=sys.stderr, flush=True)
+    return out
+
+
+def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
+    """FP64 dposv against mixed precision + FP64 iterative refinement on
+    conditioned SPD matrices (same A, same B, device resident)."""
+    L = ch.lib
+    ch.set_param(ch.TILE_SIZE, nb)
+    names = {0: "fp32", 2: "bf16x3", 1: "bf16"}
+    res = {}
+    flops = flops_dpotrf(n) + flops_dpotrs(n, nrhs)
+    with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB, ch.Desc.alloc(n, nrhs, nb) as dX:
+        def gen(kappa):
+            ch.check(L.CHAMAMD_dplgsy_cond_Tile(kappa, nrefl, dA.ptr, 1234))
+            seq = ch.Sequence()
+            ch.check(L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673, seq.ptr, seq.req))
+            ch.check(seq.wait())
+            seq.destroy()
+
+        # one untimed call sizes the refinement workspace
+        gen(kappas[0])
+        it, berr = ctypes.c_int(0), ctypes.c_double(0.0)
+        ch.check(L.CHAMAMD_dposv_mixed_Tile(ch.LOWER, dA.ptr, dB.ptr, dX.ptr, 0, ctypes.byref(it),
+                                            ctypes.byref(berr)))
+        for kappa in kappas:
+            row = {}
+            # FP64: X = B, factor and solve in place, then the backward error
+            # against a freshly generated A
+            gen(kappa)
+            seq = ch.Sequence()
+            ch.check(L.CHAMELEON_dlacpy_Tile_Async(123, dB.ptr, dX.ptr, seq.ptr, seq.req))
+            ch.check(seq.wait())
+            seq.destroy()
+            seq = ch.Sequence()
+            t0 = time.perf_counter()
+            ch.check(L.CHAMELEON_dposv_Tile_Async(ch.LOWER, dA.ptr, dX.ptr, seq.ptr, seq.req))
+            info = seq.wait()
+            t64 = time.perf_counter() - t0
+            seq.destroy()
+            gen(kappa)
+            be = ctypes.c_double(-1.0)
+            ch.check(L.CHAMAMD_Backward_Error(ch.LOWER, dA.ptr, dX.ptr, dB.ptr, ctypes.byref(be)))
+            row["fp64"] = {"ms": round(t64 * 1e3, 2), "tflops": round(flops / t64 / 1e12, 3),
+                           "berr": be.value, "info": info}
+            print(f"[bench] kappa={kappa:g} fp64 dposv {t64 * 1e3:.1f} ms berr {be.value:.3g}",
+                  file=sys.stderr, flush=True)
+            for prec in (0, 2, 1):
+                it, berr = ctypes.c_int(-99), ctypes.c_double(-2.0)
+                seq = ch.Sequence()
+                t0 = time.perf_counter()
+                rc = L.CHAMAMD_dposv_mixed_Tile_Async(ch.LOWER, dA.ptr, dB.ptr, dX.ptr, prec, ctypes.byref(it),
+                                                      ctypes.byref(berr), seq.ptr, seq.req)
+                info = seq.wait()
+                tm = time.perf_counter() - t0
+                seq.destroy()
+                ch.check(rc)
+                if it.value < 0:        # FP64 fallback overwrote A: regenerate and measure x
+                    gen(kappa)
+                    ch.check(L.CHAMAMD_Backward_Error(ch.LOWER, dA.ptr, dX.ptr, dB.ptr, ctypes.byref(berr)))
+                row[names[prec]] = {"ms": round(tm * 1e3, 2), "speedup_vs_fp64": round(t64 / tm, 3),
+                                    "iter": it.value, "berr": berr.value, "info": info,
+                                    "fp64_equiv_tflops": round(flops / tm / 1e12, 3)}
+                print(f"[bench] kappa={kappa:g} {names[prec]}+IR {tm * 1e3:.1f} ms iter {it.value} "
+                      f"berr {berr.value:.3g}", file=sys.stderr, flush=True)
+            res[f"{kappa:g}"] = row
+    return res
+
+
+def copy_probe(ch, n, nb, reps=3):
+    """Plain device copy of an n x n FP64 tile matrix (dlacpy over the tile
+    mirrors): n*n*8 bytes read and written per launch.  Run under the
+    FETCH_SIZE / WRITE_SIZE counters it gives the scale factor between the
+    counters and real bytes."""
+    L = ch.lib
+    ch.set_param(ch.TILE_SIZE, nb)
+    with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, n, nb) as dB:
+        seq = ch.Sequence()
+        ch.check(L.CHAMELEON_dplgsy_Tile_Async(float(n), 123, dA.ptr, 51, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dplgsy_Tile_Async(float(n), 123, dB.ptr, 52, seq.ptr, seq.req))
+        ch.check(seq.wait())
+        seq.destroy()
+        for _ in range(reps):
+            seq = ch.Sequence()
+            ch.check(L.CHAMELEON_dlacpy_Tile_Async(123, dA.ptr, dB.ptr, seq.ptr, seq.req))
+            ch.check(seq.wait())
+            seq.destroy()
+    print(json.dumps({"copy_probe": {"n": n, "nb": nb, "launches": reps, "bytes_read_per_launch": n * n * 8,
+                                     "bytes_written_per_launch": n * n * 8}}), flush=True)
+
+
 def cpu_baseline(threads):
     """Oracle dposv (Chameleon's OpenMP-task algorithm, OpenBLAS tile
     kernels) on the reference's CPU configuration."""
@@ -116,6 +247,10 @@ def main():
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--nb", type=int, default=0, help="tile size; 0 = pick the fastest of 256/384/512")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
+                    help="only run an N x N device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build drives exactly one GPU (asked for {args.gpus}); "
@@ -124,6 +259,9 @@ def main():
 
     import chameleon_amd as ch
     ch.init()
+    if args.copy_probe:
+        copy_probe(ch, args.copy_probe, 512)
+        return
     n, nrhs = args.n, args.nrhs
     nb = args.nb
     tune = {}
@@ -143,6 +281,8 @@ def main():
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4) if achieved else None, "traffic": None,
                 "launches": up.get("launches"),
                 "avg_launch_ms": round(up["ms"] / up["launches"], 4) if up.get("launches") else None}
+    slat = None if args.no_latency else solve_latency(ch, n, nb, 30)
+    mixed = None if args.no_mixed else mixed_leg(ch, n, nrhs, nb, (1e2, 1e4, 1e6))
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -168,6 +308,8 @@ def main():
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "tflops": round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else None}
                     for k, v in kst.items()},
+        "solve_latency_ms": slat,
+        "mixed": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

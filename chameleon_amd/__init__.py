@@ -55,6 +55,7 @@ _lib = None
 def _declare(L):
     i32, dbl, vp, u64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_ulonglong
     pp = ctypes.POINTER(ctypes.c_void_p)
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     sig = {
         "__chameleon_init": (i32, [i32, i32]),
         "__chameleon_finalize": (i32, []),
@@ -86,6 +87,7 @@ def _declare(L):
         "CHAMELEON_dlange_Tile": (dbl, [i32, vp]),
         "CHAMELEON_dlansy_Tile": (dbl, [i32, i32, vp]),
         "CHAMELEON_dlacpy_Tile": (i32, [i32, vp, vp]),
+        "CHAMELEON_dlacpy_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
         "CHAMELEON_dposv_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
         "CHAMELEON_dpotrf_Tile_Async": (i32, [i32, vp, vp, vp]),
         "CHAMELEON_dpotrs_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
@@ -97,6 +99,14 @@ def _declare(L):
         "chameleon_lapack_dposv_": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "chameleon_lapack_dpotrf_": (None, [vp, vp, vp, vp, vp]),
         "chameleon_lapack_dpotrs_": (None, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "CHAMELEON_dsposv": (i32, [i32, i32, i32, vp, i32, vp, i32, vp, i32, ip]),
+        "CHAMELEON_dsposv_Tile": (i32, [i32, vp, vp, vp, ip]),
+        "CHAMAMD_dposv_mixed_Tile": (i32, [i32, vp, vp, vp, i32, ip, dp]),
+        "CHAMAMD_dposv_mixed_Tile_Async": (i32, [i32, vp, vp, vp, i32, ip, dp, vp, vp]),
+        "CHAMAMD_dplgsy_cond_Tile": (i32, [dbl, i32, vp, u64]),
+        "CHAMAMD_Backward_Error": (i32, [i32, vp, vp, vp, dp]),
+        "CHAMAMD_Kernel_Stats": (i32, [i32, ctypes.POINTER(ctypes.c_longlong), dp, dp]),
+        "CHAMAMD_Kernel_Stats_Reset": (i32, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

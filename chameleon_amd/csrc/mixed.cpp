@@ -237,28 +237,54 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
 
 extern "C" {
 
-int CHAMAMD_dposv_mixed_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec, int *iter,
-                             double *berr)
+static int mixed_checked(const char *fn, cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
+                         int prec, int *iter, double *berr)
 {
-    int rc = check_descs("CHAMAMD_dposv_mixed_Tile", A, B, X);
+    int rc = check_descs(fn, A, B, X);
     if (rc != CHAMELEON_SUCCESS) return rc;
     if (uplo != ChamUpper && uplo != ChamLower) {
-        error("CHAMAMD_dposv_mixed_Tile", "illegal value of uplo");
+        error(fn, "illegal value of uplo");
         return -1;
     }
     if (prec < CHAMAMD_PREC_FP32 || prec > CHAMAMD_PREC_BF16X3) {
-        error("CHAMAMD_dposv_mixed_Tile", "illegal value of prec");
+        error(fn, "illegal value of prec");
         return -5;
     }
     if (A->m == 0 || B->n == 0) {
         if (iter) *iter = 0;
         return CHAMELEON_SUCCESS;
     }
-    rc = mixed_posv(uplo, A, B, X, prec, iter, berr);
+    return mixed_posv(uplo, A, B, X, prec, iter, berr);
+}
+
+int CHAMAMD_dposv_mixed_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec, int *iter,
+                             double *berr)
+{
+    int rc = mixed_checked("CHAMAMD_dposv_mixed_Tile", uplo, A, B, X, prec, iter, berr);
+    if (rc < 0 || !ctx()) return rc;
     CHAMELEON_Desc_Flush(X, nullptr);
     CHAMELEON_Desc_Flush(A, nullptr);
     CHX_HIP(hipStreamSynchronize(ctx()->s_main));
     return rc;
+}
+
+int CHAMAMD_dposv_mixed_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec,
+                                   int *iter, double *berr, RUNTIME_sequence_t *sequence,
+                                   RUNTIME_request_t *request)
+{
+    const char *fn = "CHAMAMD_dposv_mixed_Tile_Async";
+    if (!ctx()) {
+        error(fn, "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    if (!sequence || !request) {
+        error(fn, "NULL sequence or request");
+        return CHAMELEON_ERR_UNALLOCATED;
+    }
+    if (sequence->status != CHAMELEON_SUCCESS) return request_fail(sequence, request, CHAMELEON_ERR_SEQUENCE_FLUSHED);
+    request->status = CHAMELEON_SUCCESS;
+    int rc = mixed_checked(fn, uplo, A, B, X, prec, iter, berr);
+    return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
 }
 
 int CHAMAMD_Backward_Error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t *B, double *berr)

@@ -46,6 +46,16 @@ enum {
 int CHAMAMD_dposv_mixed_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec,
                              int *iter, double *berr);
 
+/* The same without the final flush of A and X to their host storage: the
+ * results stay in the device tile mirrors (read them with
+ * CHAMELEON_Desc_Flush or CHAMELEON_Desc2Lap).  The refinement loop needs
+ * host-side convergence decisions, so the call returns once the solve is
+ * complete; errors are recorded in `sequence`/`request` like the other
+ * _Tile_Async routines. */
+int CHAMAMD_dposv_mixed_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X, int prec,
+                                   int *iter, double *berr, RUNTIME_sequence_t *sequence,
+                                   RUNTIME_request_t *request);
+
 /* Symmetric positive definite matrix with eigenvalues kappa^(-i/(N-1)),
  * i < N, generated on the device: A = Q diag(lambda) Q^T with Q the
  * product of `nrefl` random Householder reflectors (seeded).  Conditioned
