@@ -72,6 +72,11 @@ struct Context {
     struct Pending { int cls; hipEvent_t a, b; double flops; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> timing_pool;
+    /* publication flags of the strip solves, tagged by launch epoch */
+    int *strip_flags = nullptr;
+    int strip_flags_n = 0;
+    int strip_epoch = 0;
+    int solve_strip_rows = 64;    /* 64 or 128 (CHAMAMD_SOLVE_STRIP) */
     long long st_launches[4] = {0, 0, 0, 0};
     double st_ms[4] = {0, 0, 0, 0};
     double st_flops[4] = {0, 0, 0, 0};
@@ -98,6 +103,7 @@ inline void timed(int cls, double flops, hipStream_t s, F body)
 
 void *workspace(size_t bytes);        /* context workspace of >= bytes */
 void *workspace2(size_t bytes);       /* second, independent workspace     */
+int *strip_flags(int n, int launches, int *epoch);
 
 Mirror *mirror_of(const CHAM_desc_t *A);
 /* Make the device mirror of A hold its current values in orientation
@@ -137,6 +143,7 @@ struct PotrfPlan {
 void potrf_run(const PotrfPlan &p);
 struct SolvePlan {
     double *TT, *TN, *BT, *W, *dinv;   /* diagonal-tile inverses, transposed RHS, one block, 64x64 inverses */
+    int R = 0;                         /* strip rows of the barrier-free solve; 0: tile-step solve */
 };
 SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info);
 void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nrp, int *d_info);
@@ -178,6 +185,8 @@ void cond_init(double *A, int NT, int ldt, int N, double kappa, hipStream_t s);
 void cond_gemv(const double *A, int NT, int ldt, int N, const double *v, double *w, hipStream_t s);
 void cond_rank2(double *A, int NT, int ldt, int N, const double *v, const double *z, hipStream_t s);
 /* solve.hip */
+int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, int dtrans, int R,
+                 double *BT, int NT, int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s);
 void solve_coop(const double *A, const double *TN, const double *TT, double *BT, double *W, int NT, int ldt, int nrhs,
                 int trans, const int *info, hipStream_t s);
 void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s);

@@ -52,6 +52,27 @@ void *workspace(size_t bytes)
     return c->ws;
 }
 
+/* Flags of the strip solves: at least `n` of them, and the first of
+ * `launches` fresh epochs (flags hold the epoch of the launch that set them,
+ * so they never need clearing between launches).  The solves run on the
+ * main stream only, so one set of flags serves every call. */
+int *strip_flags(int n, int launches, int *epoch)
+{
+    Context *c = g_ctx;
+    if (n > c->strip_flags_n || c->strip_epoch > (1 << 30)) {
+        CHX_HIP(hipStreamSynchronize(c->s_main));   /* no solve still reads the old flags */
+        if (c->strip_flags) CHX_HIP(hipFree(c->strip_flags));
+        const int cap = n > c->strip_flags_n ? n : c->strip_flags_n;
+        CHX_HIP(hipMalloc(&c->strip_flags, (size_t)cap * sizeof(int)));
+        CHX_HIP(hipMemsetAsync(c->strip_flags, 0, (size_t)cap * sizeof(int), c->s_main));
+        c->strip_flags_n = cap;
+        c->strip_epoch = 0;
+    }
+    *epoch = c->strip_epoch + 1;
+    c->strip_epoch += launches;
+    return c->strip_flags;
+}
+
 void *workspace2(size_t bytes)
 {
     Context *c = g_ctx;
@@ -304,6 +325,7 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     c->ib = env_int("CHAMELEON_INNER_BLOCK_SIZE", 48);
     c->lookahead = env_int("CHAMELEON_LOOKAHEAD", 1);
     c->translation = env_int("CHAMELEON_TRANSLATION_MODE", ChamInPlace);
+    c->solve_strip_rows = env_int("CHAMAMD_SOLVE_STRIP", 64) == 128 ? 128 : 64;
     CHX_HIP(hipSetDevice(c->device));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_update, hipStreamNonBlocking));
@@ -328,6 +350,7 @@ int __chameleon_finalize(void)
     for (hipEvent_t e : c->timing_pool) CHX_HIP(hipEventDestroy(e));
     if (c->ws) CHX_HIP(hipFree(c->ws));
     if (c->ws2) CHX_HIP(hipFree(c->ws2));
+    if (c->strip_flags) CHX_HIP(hipFree(c->strip_flags));
     if (c->pinned) CHX_HIP(hipHostFree(c->pinned));
     CHX_HIP(hipFree(c->d_info));
     CHX_HIP(hipStreamDestroy(c->s_main));

@@ -232,6 +232,25 @@ def test_large_tiles_vs_oracle(ch, n, nb, nrhs, uplo):
     assert oracle.check_solve(A0, B, oracle.dplrnt(n, nrhs, 5673))[0]
 
 
+@pytest.mark.parametrize("n,nb,nrhs", [(1536, 256, 65), (1024, 512, 130), (960, 320, 200), (1344, 192, 48)])
+def test_solve_rhs_chunks(ch, n, nb, nrhs):
+    """The strip solve handles the right-hand sides in chunks of 64 columns
+    (and a narrower last chunk); every chunk must match the oracle."""
+    say("rhs chunks", n, nb, nrhs)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    assert ch.lib.CHAMELEON_dposv(L_, n, nrhs, _p(A), n, _p(B), n) == 0
+    oracle.use_blas(True)
+    try:
+        Ar, Br, _ = reference_solution(L_, n, nrhs, nb)
+    finally:
+        oracle.use_blas(False)
+    assert oracle.rel_frobenius(B, Br) <= TOL_SOLVE
+    for j in range(0, nrhs, 16):
+        assert oracle.rel_frobenius(B[:, j:j + 16], Br[:, j:j + 16]) <= TOL_SOLVE
+
+
 @pytest.mark.parametrize("nb", [40, 80, 100, 144, 200])
 def test_partial_outer_blocks(ch, nb):
     """Tile sizes whose padded width is not a multiple of 64 exercise the
