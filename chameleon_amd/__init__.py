@@ -52,6 +52,21 @@ def build_library():
 _lib = None
 
 
+class ServerRequest(ctypes.Structure):
+    """CHAMAMD_request_t (include/chameleon_amd.h)."""
+    _fields_ = [("matrix_id", ctypes.c_longlong), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int), ("seed", ctypes.c_ulonglong),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    """CHAMAMD_server_stats_t (include/chameleon_amd.h)."""
+    _fields_ = [("completed", ctypes.c_longlong), ("rejected", ctypes.c_longlong), ("factored", ctypes.c_longlong),
+                ("reused", ctypes.c_longlong), ("resident", ctypes.c_longlong),
+                ("pool_used_bytes", ctypes.c_longlong), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("flops", ctypes.c_double)]
+
+
 def _declare(L):
     i32, dbl, vp, u64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_ulonglong
     pp = ctypes.POINTER(ctypes.c_void_p)
@@ -107,6 +122,30 @@ def _declare(L):
         "CHAMAMD_Backward_Error": (i32, [i32, vp, vp, vp, dp]),
         "CHAMAMD_Kernel_Stats": (i32, [i32, ctypes.POINTER(ctypes.c_longlong), dp, dp]),
         "CHAMAMD_Kernel_Stats_Reset": (i32, []),
+        "CHAMELEON_dplgsy_batch_Tile": (i32, [dbl, vp, u64]),
+        "CHAMELEON_dpotrf_batch_Tile": (i32, [i32, vp]),
+        "CHAMAMD_Batch_Create": (i32, [pp, i32, ip, ip]),
+        "CHAMAMD_Batch_Destroy": (i32, [pp]),
+        "CHAMAMD_Batch_SetA": (i32, [vp, i32, i32, vp, i32]),
+        "CHAMAMD_Batch_GetA": (i32, [vp, i32, i32, vp, i32]),
+        "CHAMAMD_Batch_SetB": (i32, [vp, i32, vp, i32]),
+        "CHAMAMD_Batch_GetB": (i32, [vp, i32, vp, i32]),
+        "CHAMAMD_dplgsy_batch": (i32, [vp, u64]),
+        "CHAMAMD_dplrnt_batch": (i32, [vp, u64]),
+        "CHAMAMD_dpotrf_batch": (i32, [vp, ip]),
+        "CHAMAMD_dpotrs_batch": (i32, [vp]),
+        "CHAMAMD_dposv_batch": (i32, [vp, ip]),
+        "CHAMAMD_dpotrf_batch_Async": (i32, [vp, vp, vp]),
+        "CHAMAMD_dpotrs_batch_Async": (i32, [vp, vp, vp]),
+        "CHAMAMD_dposv_batch_Async": (i32, [vp, vp, vp]),
+        "CHAMAMD_Batch_Info": (i32, [vp, ip]),
+        "CHAMAMD_Server_Create": (i32, [pp, ctypes.c_size_t]),
+        "CHAMAMD_Server_Destroy": (i32, [pp]),
+        "CHAMAMD_Server_Submit": (i32, [vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(ctypes.c_longlong)]),
+        "CHAMAMD_Server_Progress": (i32, [vp, ip]),
+        "CHAMAMD_Server_Test": (i32, [vp, ctypes.c_longlong, ip, ip]),
+        "CHAMAMD_Server_Stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
+        "CHAMAMD_Server_Reset_Stats": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -248,3 +287,54 @@ def header_functions():
                     continue
                 names.add(name)
     return names
+
+
+class Server:
+    """Owning wrapper of a CHAMAMD_server_t*.  Keeps the right-hand-side
+    arrays of submitted requests alive until they are served."""
+
+    def __init__(self, budget_bytes):
+        self.ptr = ctypes.c_void_p()
+        check(load().CHAMAMD_Server_Create(ctypes.byref(self.ptr), int(budget_bytes)), "CHAMAMD_Server_Create")
+        self.keep = {}
+
+    def submit(self, matrix_id, n, B, seed=0, A=None):
+        """B: Fortran float64 array (n x nrhs), overwritten by the solution."""
+        assert B.dtype == np.float64 and B.flags.f_contiguous
+        r = ServerRequest(matrix_id, n, B.shape[1], A.ctypes.data if A is not None else None,
+                          A.shape[0] if A is not None else 0, seed, B.ctypes.data, B.shape[0])
+        t = ctypes.c_longlong()
+        check(load().CHAMAMD_Server_Submit(self.ptr, ctypes.byref(r), ctypes.byref(t)), "CHAMAMD_Server_Submit")
+        self.keep[t.value] = (A, B)
+        return t.value
+
+    def progress(self):
+        n = ctypes.c_int()
+        check(load().CHAMAMD_Server_Progress(self.ptr, ctypes.byref(n)), "CHAMAMD_Server_Progress")
+        return n.value
+
+    def test(self, ticket):
+        d, st = ctypes.c_int(), ctypes.c_int()
+        check(load().CHAMAMD_Server_Test(self.ptr, ticket, ctypes.byref(d), ctypes.byref(st)))
+        if d.value:
+            self.keep.pop(ticket, None)
+        return bool(d.value), st.value
+
+    def stats(self):
+        st = ServerStats()
+        check(load().CHAMAMD_Server_Stats(self.ptr, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        check(load().CHAMAMD_Server_Reset_Stats(self.ptr))
+
+    def destroy(self):
+        if self.ptr:
+            load().CHAMAMD_Server_Destroy(ctypes.byref(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()

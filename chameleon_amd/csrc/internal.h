@@ -77,6 +77,7 @@ struct Context {
     int strip_flags_n = 0;
     int strip_epoch = 0;
     int solve_strip_rows = 64;    /* 64 or 128 (CHAMAMD_SOLVE_STRIP) */
+    int sched = 0;                /* lookahead issue order (CHAMAMD_SCHED) */
     long long st_launches[4] = {0, 0, 0, 0};
     double st_ms[4] = {0, 0, 0, 0};
     double st_flops[4] = {0, 0, 0, 0};
@@ -148,6 +149,18 @@ struct SolvePlan {
 SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info);
 void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nrp, int *d_info);
 
+/* one system of a batched generation (kernels/batch.hip) */
+struct BatchGenSys {
+    long long off;                 /* first element of the system                    */
+    long long base;                /* origin of the element rule                     */
+    unsigned long long seed, abig, cbig;   /* abig, cbig: the LCG map of bigM steps  */
+    double bump;
+    int ld, rows, cols;            /* leading dimension; rows / columns written      */
+    int nvr, nvc;                  /* valid rows / columns (the rest is padding)     */
+    int bigM;
+    int pad_one;                   /* plgsy: 1 on the padded diagonal (else 0)       */
+};
+
 /* ---- kernel launchers (kernels directory) ---- */
 namespace k {
 /* util.hip */
@@ -172,6 +185,17 @@ void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, i
                                     hipStream_t s);
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s);
 void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
+void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, const int *nvalid, double *dinv,
+                const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s);
+/* batch.hip */
+void batch_gen(double *A, const BatchGenSys *sys, const long long *unit0, long long units, int count, int sym,
+               hipStream_t s);
+void batch_solve(const double *A, const long long *aoff, const int *np, const int *ld, const double *dinv,
+                 const long long *doff, double *B, const long long *boff, const int *nrhs, const int *info, int count,
+                 int *queue, int grid, hipStream_t s);
+void zero_ints(int *p, int n, hipStream_t s);
+void tile_pad_diag(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N, int mode,
+                   hipStream_t s);
 /* mixed.hip */
 void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int mode, hipStream_t s);
 void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, int n0, int n1, int NT, const int *info,

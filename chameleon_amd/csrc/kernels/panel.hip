@@ -168,43 +168,71 @@ __device__ void assemble_inverse(const double (*Ld)[OBP], const double (*Lb)[16]
 }
 
 /* ================================================================== */
-/* POTRF of one diagonal tile, one workgroup of 8 wavefronts.
+/* POTRF of one matrix held in HBM per job, one workgroup of 8 wavefronts
+ * per job: the diagonal tile of the tiled factorisation (one job), or
+ * every system of a batch (jobs taken from a queue).
  * For each 64-column block C (left-looking):
- *   (a) rows [C, ldt) of the block -= T[., 0:C] T[C:C+64, 0:C]^T   (MFMA)
+ *   (a) rows [C, np) of the block -= T[., 0:C] T[C:C+64, 0:C]^T   (MFMA)
  *   (b) factor the 64 x 64 diagonal block in LDS by 16-column steps
  *   (c) build its inverse, store it in dinv
  *   (d) rows below: X = P Linv^T                                  (MFMA)
- * The strictly upper triangle of the tile is never written.  info:
- * iinfo + (1-based column of the first non-positive pivot), only for
- * pivots inside the valid part of the tile (nvalid columns).            */
+ * The strictly upper triangle of the matrix is never written.  A failed
+ * job reports the 1-based column of its first non-positive pivot, only
+ * for pivots inside its valid part (nvalid columns).                    */
 #define POTRF_THREADS 512
 #define POTRF_WAVES 8
-__global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(double *T, int ldt, double *dinv, int iinfo,
-                                                                    int nvalid, int *info)
+
+struct PotrfJobs {
+    double *A;               /* job j: matrix at A + off[j] (off null: A)               */
+    const long long *off;
+    const int *np;           /* order (multiple of 16); null: np0                        */
+    const int *ld;           /* leading dimension; null: ld0, or np when ld0 = 0         */
+    const int *nvalid;       /* valid columns; null: nvalid0                             */
+    double *dinv;            /* 64 x 64 inverses: job j at dinv + doff[j] (null: dinv)   */
+    const long long *doff;
+    int np0, ld0, nvalid0;
+    int count;               /* number of jobs                                           */
+    int *queue;              /* job counter (batches); null: job = blockIdx.x            */
+    int *info;               /* single job: *info = iinfo + column on failure;
+                              * batches: info[j] = column or 0                           */
+    int iinfo, batch;
+};
+
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_jobs_kernel(PotrfJobs J)
 {
     __shared__ double Ld[OB][OBP];
     __shared__ double Lv[OB][OBP];
     __shared__ double Lb[4][16][17];
     __shared__ double Rd[16];
     __shared__ double Ys[2][16 * OB];   /* Y slice of phase (a): 64 rows x 16 k, staged per slice */
-    __shared__ int s_fail;
+    __shared__ int s_fail, s_job;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    if (*info != 0) return;
-    /* this kernel is on the critical path of the factorisation: its waves
-     * win issue arbitration against co-resident trailing-update waves */
+    if (!J.batch && *J.info != 0) return;
+    /* the single-job form is on the critical path of the factorisation:
+     * its waves win issue arbitration against co-resident update waves */
     __builtin_amdgcn_s_setprio(2);
+  for (int pass = 0;; ++pass) {
+    if (tid == 0) s_job = J.queue ? atomicAdd(J.queue, 1) : (pass == 0 ? (int)blockIdx.x : J.count);
+    __syncthreads();
+    const int job = s_job;
+    if (job >= J.count) break;
+    double *T = J.A + (J.off ? J.off[job] : 0);
+    double *dinv = J.dinv + (J.doff ? J.doff[job] : 0);
+    const int np = J.np ? J.np[job] : J.np0;
+    const int ldt = J.ld ? J.ld[job] : (J.ld0 ? J.ld0 : np);
+    const int nvalid = J.nvalid ? J.nvalid[job] : J.nvalid0;
     if (tid == 0) s_fail = 0;
     __syncthreads();
 
-    for (int C = 0; C < ldt; C += OB) {
-        const int W = min(OB, ldt - C), nfb = W >> 4;
-        /* (a) rows [C, ldt) x cols [C, C+W) -= T[., 0:C] T[C:C+W, 0:C]^T.
+    for (int C = 0; C < np; C += OB) {
+        const int W = min(OB, np - C), nfb = W >> 4;
+        /* (a) rows [C, np) x cols [C, C+W) -= T[., 0:C] T[C:C+W, 0:C]^T.
          * The 16-deep slices of the W shared rows (Y) are staged in LDS,
          * double-buffered, so one CU's L2 bandwidth carries each slice
          * once; every wavefront streams its own 16 rows (X). */
         if (C > 0) {
-            const int nrb = (ldt - C) >> 4;
+            const int nrb = (np - C) >> 4;
             for (int rb0 = 0; rb0 < nrb; rb0 += POTRF_WAVES) {
                 const int rb = rb0 + wave;
                 const bool active = rb < nrb;
@@ -333,9 +361,9 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(double *T, in
             if (r < W && c < W && r >= c) T[(C + r) + (size_t)(C + c) * ldt] = Ld[r][c];
             dv[e] = Lv[r][c];
         }
-        /* (d) rows [C+W, ldt): X = P Lv^T, D = Lv P^T */
+        /* (d) rows [C+W, np): X = P Lv^T, D = Lv P^T */
         {
-            const int nrb = (ldt - C - W) >> 4;
+            const int nrb = (np - C - W) >> 4;
             for (int rb = wave; rb < nrb; rb += POTRF_WAVES) {
                 const int r0 = C + W + rb * 16;
                 double pv[16];
@@ -362,12 +390,49 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_diag_kernel(double *T, in
         }
         __syncthreads();
     }
-    if (tid == 0 && s_fail > 0) *info = iinfo + s_fail;
+    if (tid == 0) {
+        if (J.batch) J.info[job] = s_fail > 0 ? s_fail : 0;
+        else if (s_fail > 0) *J.info = J.iinfo + s_fail;
+    }
+    __syncthreads();
+  }
 }
 
 void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, hipStream_t s)
 {
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(POTRF_THREADS), 0, s, Akk, ldt, dinv, iinfo, nvalid, info);
+    PotrfJobs J = {};
+    J.A = Akk; J.dinv = dinv;
+    J.np0 = ldt; J.ld0 = ldt; J.nvalid0 = nvalid;
+    J.count = 1; J.info = info; J.iinfo = iinfo; J.batch = 0;
+    hipLaunchKernelGGL(potrf_jobs_kernel, dim3(1), dim3(POTRF_THREADS), 0, s, J);
+}
+
+/* Batched form: `count` matrices, job j of order np[j] (multiple of 16)
+ * at A + off[j] with leading dimension ld[j] (null: np[j]), inverses at
+ * dinv + doff[j]; info[j] = first failing column or 0.  `queue` is a
+ * zeroed device counter; jobs are taken in index order by `grid`
+ * persistent workgroups (grid <= 0: as many as are resident at once; sort
+ * the jobs largest first for balance). */
+void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, const int *nvalid, double *dinv,
+                const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s)
+{
+    PotrfJobs J = {};
+    J.A = A; J.off = off; J.np = np; J.ld = ld; J.nvalid = nvalid; J.dinv = dinv; J.doff = doff;
+    J.count = count; J.queue = queue; J.info = info; J.batch = 1;
+    if (count <= 0) return;
+    if (grid <= 0) {
+        static int resident = 0;
+        if (!resident) {
+            int dev = 0, ncu = 0, per = 0;
+            CHX_HIP(hipGetDevice(&dev));
+            CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, potrf_jobs_kernel, POTRF_THREADS, 0));
+            resident = ncu * (per > 0 ? per : 1);
+        }
+        grid = resident;
+    }
+    if (grid > count) grid = count;
+    hipLaunchKernelGGL(potrf_jobs_kernel, dim3(grid), dim3(POTRF_THREADS), 0, s, J);
 }
 
 /* ================================================================== */

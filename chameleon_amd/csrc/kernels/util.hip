@@ -3,6 +3,7 @@
  * and orientation, the dplgsy/dplrnt generators, tile copies and norms.
  */
 #include "../internal.h"
+#include "rng.h"
 
 namespace chx {
 namespace k {
@@ -110,26 +111,7 @@ void transpose_tiles_inplace(double *d, int nt, int ldt, hipStream_t s)
  * column of one tile: one jump-ahead, then single steps along the column
  * (pos + 1) or, where the position runs across rows, steps of bigM
  * positions with the precomputed affine map (abig, cbig).                */
-#define RND_A 6364136223846793005ULL
-#define RND_C 1ULL
 #define RUN 32
-
-__device__ __forceinline__ unsigned long long rnd_jump(unsigned long long n, unsigned long long x)
-{
-    unsigned long long a = RND_A, c = RND_C;
-    while (n) {
-        if (n & 1ULL) x = a * x + c;
-        c = c * (a + 1ULL);
-        a = a * a;
-        n >>= 1;
-    }
-    return x;
-}
-
-__device__ __forceinline__ double rnd_val(unsigned long long x)
-{
-    return 0.5 - (double)x * 5.4210108624275222e-20;
-}
 
 struct GenArgs {
     Geo g;

@@ -52,6 +52,14 @@ int CHAMELEON_dlansy_Tile_Async(cham_normtype_t norm, cham_uplo_t uplo, CHAM_des
 int CHAMELEON_dlacpy_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B,
                                 RUNTIME_sequence_t *sequence, RUNTIME_request_t *request);
 
+/* Tile batches: every tile of A is an independent matrix of the order of
+ * its tile column.  dplgsy_batch: compute/zplghe_batch.c:103 (tile (m,n)
+ * generated at origin (n*mt + m)*mb*nb with bigM = 0); dpotrf_batch:
+ * compute/zpotrf_batch.c:89 (each tile factored in place; returns the
+ * first failing tile's column where the reference only prints it). */
+int CHAMELEON_dplgsy_batch_Tile(double bump, CHAM_desc_t *A, unsigned long long int seed);
+int CHAMELEON_dpotrf_batch_Tile(cham_uplo_t uplo, CHAM_desc_t *A);
+
 #ifdef __cplusplus
 }
 #endif

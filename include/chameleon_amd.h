@@ -5,6 +5,7 @@
 #ifndef CHAMELEON_AMD_EXTRAS_H
 #define CHAMELEON_AMD_EXTRAS_H
 
+#include <stddef.h>
 #include "chameleon.h"
 
 #ifdef __cplusplus
@@ -67,6 +68,77 @@ int CHAMAMD_dplgsy_cond_Tile(double kappa, int nrefl, CHAM_desc_t *A, unsigned l
 /* The same backward error for any X, A symmetric given by its `uplo`
  * triangle, computed on the device in FP64. */
 int CHAMAMD_Backward_Error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t *B, double *berr);
+
+/* ------------------------------------------------------------------ */
+/* Uneven batches: `count` independent SPD systems A_i X_i = B_i of orders
+ * n[i] with nrhs[i] right-hand sides, resident on the device.  No
+ * counterpart in the reference (its batch routines take one descriptor of
+ * equal tiles and run on the CPU only, compute/zpotrf_batch.c:58-64).
+ * Each system is factored and solved by one workgroup, largest systems
+ * first; its result is bitwise independent of the rest of the batch.
+ * Factors are kept in Lower storage; Set/GetA take either triangle. */
+typedef struct CHAMAMD_batch_s CHAMAMD_batch_t;
+
+int CHAMAMD_Batch_Create(CHAMAMD_batch_t **batch, int count, const int *n, const int *nrhs);
+int CHAMAMD_Batch_Destroy(CHAMAMD_batch_t **batch);
+/* host <-> device copies of system i (column-major, leading dimension ld) */
+int CHAMAMD_Batch_SetA(CHAMAMD_batch_t *batch, int i, cham_uplo_t uplo, const double *A, int lda);
+int CHAMAMD_Batch_GetA(CHAMAMD_batch_t *batch, int i, cham_uplo_t uplo, double *A, int lda);
+int CHAMAMD_Batch_SetB(CHAMAMD_batch_t *batch, int i, const double *B, int ldb);
+int CHAMAMD_Batch_GetB(CHAMAMD_batch_t *batch, int i, double *B, int ldb);
+/* on the device: A_i = dplgsy(bump = n_i, n_i x n_i, seed + i) and
+ * B_i = dplrnt(n_i x nrhs_i, seed + i) (compute/zplgsy.c, zplrnt.c) */
+int CHAMAMD_dplgsy_batch(CHAMAMD_batch_t *batch, unsigned long long int seed);
+int CHAMAMD_dplrnt_batch(CHAMAMD_batch_t *batch, unsigned long long int seed);
+/* factor / solve / both; the blocking forms fill info[i] (may be NULL)
+ * with the first non-positive pivot column of system i (0: success) and
+ * return the first nonzero one in system order.  Systems that failed to
+ * factor are left out of the solve. */
+int CHAMAMD_dpotrf_batch(CHAMAMD_batch_t *batch, int *info);
+int CHAMAMD_dpotrs_batch(CHAMAMD_batch_t *batch);
+int CHAMAMD_dposv_batch(CHAMAMD_batch_t *batch, int *info);
+int CHAMAMD_dpotrf_batch_Async(CHAMAMD_batch_t *batch, RUNTIME_sequence_t *sequence, RUNTIME_request_t *request);
+int CHAMAMD_dpotrs_batch_Async(CHAMAMD_batch_t *batch, RUNTIME_sequence_t *sequence, RUNTIME_request_t *request);
+int CHAMAMD_dposv_batch_Async(CHAMAMD_batch_t *batch, RUNTIME_sequence_t *sequence, RUNTIME_request_t *request);
+/* per-system info of the last factorisation (after the sequence completed) */
+int CHAMAMD_Batch_Info(CHAMAMD_batch_t *batch, int *info);
+
+/* ------------------------------------------------------------------ */
+/* Solve server: requests A X = B arrive over time; factors stay resident
+ * in a device pool of a fixed HBM budget (least recently used factors not
+ * needed by pending requests are evicted), and each Progress call serves
+ * everything pending as one batched round.  No counterpart in the
+ * reference beyond its sequence/request interface
+ * (include/chameleon/runtime_struct.h:78-103), which has no admission
+ * control, residency or latency accounting.  Host-thread model as the
+ * rest of the API: one thread calls in. */
+typedef struct CHAMAMD_server_s CHAMAMD_server_t;
+typedef struct {
+    long long matrix_id;      /* requests with the same id share one matrix and its factor */
+    int n, nrhs;
+    const double *A;          /* host matrix (lower triangle read, column-major, lda), or NULL: */
+    int lda;                  /*   A = dplgsy(bump = n, n x n, seed) generated on the device    */
+    unsigned long long seed;
+    double *B;                /* host right-hand sides in, solution out (column-major, ldb)   */
+    int ldb;
+} CHAMAMD_request_t;
+typedef struct {
+    long long completed, rejected, factored, reused, resident, pool_used_bytes;
+    double p50_ms, p99_ms, max_ms;   /* submit-to-solution latency of completed requests      */
+    double flops;                    /* factor + solve flops served (Chameleon's flop model)   */
+} CHAMAMD_server_stats_t;
+
+int CHAMAMD_Server_Create(CHAMAMD_server_t **server, size_t hbm_budget_bytes);
+int CHAMAMD_Server_Destroy(CHAMAMD_server_t **server);
+int CHAMAMD_Server_Submit(CHAMAMD_server_t *server, const CHAMAMD_request_t *req, long long *ticket);
+/* serve everything pending (blocking); *processed = requests taken */
+int CHAMAMD_Server_Progress(CHAMAMD_server_t *server, int *processed);
+/* *done = 1 once the request finished; *status = 0, the not-positive-
+ * definite column, or a negative error (CHAMELEON_ERR_OUT_OF_RESOURCES
+ * when its factor cannot fit the budget) */
+int CHAMAMD_Server_Test(CHAMAMD_server_t *server, long long ticket, int *done, int *status);
+int CHAMAMD_Server_Stats(CHAMAMD_server_t *server, CHAMAMD_server_stats_t *stats);
+int CHAMAMD_Server_Reset_Stats(CHAMAMD_server_t *server);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,8 @@ def lib():
         L.orc_dpotrs.argtypes = [i32, i32, i32, ptr, i32, ptr, i32, i32, i32]
         L.orc_dposv.argtypes = [i32, i32, i32, ptr, i32, ptr, i32, i32, i32]
         L.orc_set_blas.argtypes = [ptr, ptr, ptr, ptr, ptr]
+        L.orc_dplgsy_batch.argtypes = [dbl, i32, i32, i32, i32, ptr, i32, u64]
+        L.orc_dpotrf_batch.argtypes = [i32, i32, i32, i32, i32, ptr, i32, ptr]
         L.orc_wtime.restype = dbl
         _lib = L
     return _lib
@@ -88,6 +90,25 @@ def dplrnt(m, n, seed, lda=None, nb=None, out=None):
     A = out if out is not None else _fortran((lda, n))
     lib().orc_dplrnt(m, n, _p(A), lda, nb or max(1, m), seed)
     return A
+
+
+def dplgsy_batch(bump, M, N, mb, nb, seed, lda=None):
+    """Tile batch of CHAMELEON_dplgsy_batch_Tile: every mb x nb tile of an
+    M x N column-major matrix is an independent symmetric matrix."""
+    lda = lda or max(1, M)
+    A = _fortran((lda, N))
+    lib().orc_dplgsy_batch(float(bump), M, N, mb, nb, _p(A), lda, seed)
+    return A
+
+
+def dpotrf_batch(uplo, A, M, N, mb, nb):
+    """CHAMELEON_dpotrf_batch_Tile on a column-major matrix: every tile
+    factored alone; returns the per-tile info array (tile n*mt + m)."""
+    assert A.flags.f_contiguous and A.dtype == np.float64
+    mt, nt = -(-M // mb), -(-N // nb)
+    info = np.zeros(mt * nt, dtype=np.int32)
+    lib().orc_dpotrf_batch(uplo, M, N, mb, nb, _p(A), A.shape[0], _p(info))
+    return info
 
 
 def dpotrf(uplo, A, n, nb, nthreads=0):

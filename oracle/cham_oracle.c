@@ -18,6 +18,8 @@
  *   orc_pdpotrf                      compute/pzpotrf.c:84-193 (right-looking task loops)
  *   orc_pdtrsm_left                  compute/pztrsm.c:60-201 (the four Left branches)
  *   orc_dpotrf/dpotrs/dposv          compute/zpotrf.c, zpotrs.c, zposv.c (CM in-place layout)
+ *   orc_dplgsy_batch                 compute/zplghe_batch.c:36-66 (tile origin (n*mt+m)*mb*nb, bigM 0)
+ *   orc_dpotrf_batch                 compute/zpotrf_batch.c:28-49 (every tile factored alone)
  * Tasks are OpenMP tasks with depend() clauses on the tiles they read and
  * write, the scheme of runtime/openmp/codelets/codelet_z{potrf,trsm,syrk,gemm}.c.
  *
@@ -462,6 +464,36 @@ int orc_dposv(int uplo, int N, int NRHS, double *A, int lda, double *B, int ldb,
     pdtrsm_left(uplo, uplo == Upper ? Trans : NoTrans, &dA, &dB);
     pdtrsm_left(uplo, uplo == Upper ? NoTrans : Trans, &dA, &dB);
     return 0;
+}
+
+/* ------------------------------------------------------------ batches */
+/* CHAMELEON_zplghe_batch_Tile (compute/zplghe_batch.c:36-66, d instance):
+ * every tile (m, n) of an M x N matrix cut in mb x nb tiles is an
+ * independent tempnn x tempnn matrix generated with origin
+ * m0 = n0 = (n*mt + m)*mb*nb and bigM = 0.  CM layout: tile (m, n) at
+ * A + n*nb*lda + m*mb. */
+void orc_dplgsy_batch(double bump, int M, int N, int mb, int nb, double *A, int lda, uint64_t seed)
+{
+    int mt = (M + mb - 1) / mb, nt = (N + nb - 1) / nb;
+    for (int n = 0; n < nt; n++)
+        for (int m = 0; m < mt; m++) {
+            int tempnn = tdim(n, nb, N);
+            int m0 = (n * mt + m) * (mb * nb);
+            orc_dplgsy_tile(bump, tempnn, tempnn, A + (size_t)n * nb * lda + (size_t)m * mb, lda, 0, m0, m0, seed);
+        }
+}
+
+/* CHAMELEON_zpotrf_batch_Tile (compute/zpotrf_batch.c:28-49): every tile
+ * factored in place by the tile kernel; info[n*mt + m] receives each
+ * tile's result (the reference prints failures and returns 0). */
+void orc_dpotrf_batch(int uplo, int M, int N, int mb, int nb, double *A, int lda, int *info)
+{
+    int mt = (M + mb - 1) / mb, nt = (N + nb - 1) / nb;
+    for (int n = 0; n < nt; n++)
+        for (int m = 0; m < mt; m++) {
+            int tempnn = tdim(n, nb, N);
+            info[n * mt + m] = orc_dpotrf_tile(uplo, tempnn, A + (size_t)n * nb * lda + (size_t)m * mb, lda);
+        }
 }
 
 double orc_wtime(void) { return omp_get_wtime(); }

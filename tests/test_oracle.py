@@ -164,3 +164,41 @@ def test_checks_catch_planted_errors():
     Xb = X.copy()
     Xb[5, 1] *= 1 + 1e-10
     assert not oracle.check_solve(A0, Xb, B)[0]
+
+
+# ------------------------------------------------------------------ batches
+@pytest.mark.parametrize("M,N,mb,nb", [(96, 64, 32, 32), (40, 40, 20, 20), (48, 16, 16, 16)])
+def test_batch_generator_closed_form(M, N, mb, nb):
+    """zplghe_batch.c:56-57 puts tile (m,n) at origin m0 = n0 = (n*mt+m)*mb*nb
+    with bigM = 0, so element (i,j) of a tile is the LCG value at position
+    m0 + max(i,j) (diagonal + bump): checked against the jump-ahead alone."""
+    seed, bump = 77, 3.0
+    A = oracle.dplgsy_batch(bump, M, N, mb, nb, seed)
+    mt = M // mb
+    two64 = 2.0 ** -64
+    for n in range(N // nb):
+        for m in range(mt):
+            m0 = (n * mt + m) * mb * nb
+            T = A[m * mb:(m + 1) * mb, n * nb:(n + 1) * nb]
+            for i in range(nb):
+                for j in range(nb):
+                    v = 0.5 - oracle.rnd64_jump(m0 + max(i, j), seed) * two64
+                    assert T[i, j] == v + (bump if i == j else 0.0)
+
+
+def test_batch_potrf_tiles_are_independent():
+    M, N, mb, nb = 90, 60, 30, 30
+    A = oracle.dplgsy_batch(float(nb), M, N, mb, nb, 5)
+    A0 = A.copy(order="F")
+    info = oracle.dpotrf_batch(L, A, M, N, mb, nb)
+    assert not info.any()
+    for n in range(N // nb):
+        for m in range(M // mb):
+            T0 = A0[m * mb:(m + 1) * mb, n * nb:(n + 1) * nb]
+            T = A[m * mb:(m + 1) * mb, n * nb:(n + 1) * nb]
+            Lr = np.linalg.cholesky(T0)          # each tile alone, textbook
+            assert np.allclose(np.tril(T), Lr, rtol=0, atol=1e-13)
+            assert np.array_equal(np.triu(T, 1), np.triu(T0, 1))   # upper triangle untouched
+    bad = oracle.dplgsy_batch(1.0, M, N, mb, nb, 5)
+    info = oracle.dpotrf_batch(L, bad, M, N, mb, nb)
+    assert (info > 0).all()
