@@ -25,6 +25,14 @@ Reported (one JSON line):
                refinement (FP32, split-BF16 and BF16 trailing products) --
                time, speed-up over FP64, refinement steps and the final FP64
                backward error ||b - A x||_inf / (||A||_inf ||x||_inf N eps)
+  batch        20 000 independent SPD systems, n = floor(exp(U(ln 16, ln 1024))),
+               nrhs uniform in 1..8, A_i = dplgsy(bump = n_i, seed + i),
+               B_i = dplrnt(seed + i), generated on the device: time of the
+               batched factor + solve and its flop rate
+  serve        Poisson arrivals (2000 requests/s for a few seconds), n log-uniform
+               in [64, 8192], nrhs in {1, 4, 16}, 30 % of requests reusing an
+               earlier matrix, 200 GB factor budget: sustained request rate and
+               flop rate, p50 / p99 latency from scheduled arrival to solution
 """
 import argparse
 import ctypes
@@ -171,6 +179,8 @@ def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
                   file=sys.stderr, flush=True)
             for prec in (0, 2, 1):
                 it, berr = ctypes.c_int(-99), ctypes.c_double(-2.0)
+                L.CHAMELEON_Enable(6)       # CHAMELEON_GENERATE_STATS
+                L.CHAMAMD_Kernel_Stats_Reset()
                 seq = ch.Sequence()
                 t0 = time.perf_counter()
                 rc = L.CHAMAMD_dposv_mixed_Tile_Async(ch.LOWER, dA.ptr, dB.ptr, dX.ptr, prec, ctypes.byref(it),
@@ -178,17 +188,119 @@ def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
                 info = seq.wait()
                 tm = time.perf_counter() - t0
                 seq.destroy()
+                kst = {}
+                for cls, name in {0: "update", 1: "potrf", 2: "trsm", 3: "solve"}.items():
+                    la, ms, fl = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+                    L.CHAMAMD_Kernel_Stats(cls, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl))
+                    kst[name] = {"launches": la.value, "ms": round(ms.value, 2),
+                                 "tflops": round(fl.value / (ms.value * 1e-3) / 1e12, 2) if ms.value else None}
+                L.CHAMELEON_Disable(6)
                 ch.check(rc)
                 if it.value < 0:        # FP64 fallback overwrote A: regenerate and measure x
                     gen(kappa)
                     ch.check(L.CHAMAMD_Backward_Error(ch.LOWER, dA.ptr, dX.ptr, dB.ptr, ctypes.byref(berr)))
                 row[names[prec]] = {"ms": round(tm * 1e3, 2), "speedup_vs_fp64": round(t64 / tm, 3),
                                     "iter": it.value, "berr": berr.value, "info": info,
-                                    "fp64_equiv_tflops": round(flops / tm / 1e12, 3)}
+                                    "fp64_equiv_tflops": round(flops / tm / 1e12, 3), "kernels": kst}
                 print(f"[bench] kappa={kappa:g} {names[prec]}+IR {tm * 1e3:.1f} ms iter {it.value} "
                       f"berr {berr.value:.3g}", file=sys.stderr, flush=True)
             res[f"{kappa:g}"] = row
     return res
+
+
+def batch_leg(ch, count=20000, reps=3, seed=2024):
+    """Uneven batch: every system factored and solved by one workgroup."""
+    L = ch.lib
+    rng = np.random.default_rng(11)
+    n = np.floor(np.exp(rng.uniform(np.log(16), np.log(1024), count))).astype(np.int32)
+    nrhs = rng.integers(1, 9, count).astype(np.int32)
+    nn = n.astype(np.float64)
+    flops = float(np.sum(nn * (nn * nn / 3.0 + nn / 2.0 + 1.0 / 6.0) + 2.0 * nn * nn * nrhs))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    b = ctypes.c_void_p()
+    ch.check(L.CHAMAMD_Batch_Create(ctypes.byref(b), count, ip(n), ip(nrhs)))
+    times, bad = [], 0
+    try:
+        for it in range(reps + 1):
+            ch.check(L.CHAMAMD_dplgsy_batch(b, seed))
+            ch.check(L.CHAMAMD_dplrnt_batch(b, seed))
+            info = np.zeros(count, dtype=np.int32)
+            t0 = time.perf_counter()
+            rc = L.CHAMAMD_dposv_batch(b, ip(info))
+            t1 = time.perf_counter()
+            bad = int(np.count_nonzero(info))
+            if rc != 0 or bad:
+                raise RuntimeError(f"batch dposv rc={rc}, {bad} failing systems")
+            if it > 0:
+                times.append(t1 - t0)
+            print(f"[bench] batch {count} systems: {1e3 * (t1 - t0):.1f} ms", file=sys.stderr, flush=True)
+    finally:
+        L.CHAMAMD_Batch_Destroy(ctypes.byref(b))
+    best = min(times)
+    return {"systems": count, "n_range": [int(n.min()), int(n.max())], "flops": flops,
+            "ms": round(best * 1e3, 2), "tflops": round(flops / best / 1e12, 3),
+            "ms_all": [round(t * 1e3, 2) for t in times]}
+
+
+def serve_leg(ch, seconds, rate=2000.0, budget=200 << 30, seed=7, drain=20.0):
+    """Open-loop arrivals against the solve server; latency counted from
+    each request's scheduled arrival time (queueing included).  Arrivals
+    stop after `seconds`; serving stops `drain` seconds later and requests
+    still unserved then are reported (their latency is unknown, so the
+    percentiles are given only when fewer than 1 % are missing)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    reqs, seen = [], []
+    for i in range(len(arrivals)):
+        if seen and rng.random() < 0.3:
+            mid, n = seen[rng.integers(len(seen))]
+        else:
+            n = int(np.floor(np.exp(rng.uniform(np.log(64), np.log(8192)))))
+            mid = i
+            seen.append((mid, n))
+        reqs.append((float(arrivals[i]), mid, n, int(rng.choice([1, 4, 16]))))
+    srv = ch.Server(budget)
+    lat, done_at = [], {}
+    try:
+        t0 = time.perf_counter()
+        nxt, live = 0, {}
+        deadline = seconds + drain
+        while (nxt < len(reqs) or live) and time.perf_counter() - t0 < deadline:
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt][0] <= now:
+                ta, mid, n, r = reqs[nxt]
+                B = np.asfortranarray(rng.standard_normal((n, r)))
+                live[srv.submit(mid, n, B, seed=1000 + mid)] = ta
+                nxt += 1
+            if live:
+                srv.progress()
+                tdone = time.perf_counter() - t0
+                for t in list(live):
+                    ok, st = srv.test(t)
+                    if ok:
+                        if st != 0:
+                            raise RuntimeError(f"server request status {st}")
+                        lat.append(tdone - live.pop(t))
+            elif nxt < len(reqs):
+                time.sleep(max(0.0, reqs[nxt][0] - (time.perf_counter() - t0)))
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.destroy()
+    lat = np.array(lat) * 1e3
+    served = len(lat)
+    unserved = len(reqs) - served
+    full = unserved <= 0.01 * len(reqs)
+    out = {"requests": len(reqs), "served": served, "unserved": unserved, "offered_rate": rate,
+           "window_s": seconds, "wall_s": round(wall, 3), "served_rate": round(served / wall, 1),
+           "tflops": round(st["flops"] / wall / 1e12, 3),
+           "p50_ms": round(float(np.percentile(lat, 50)), 2) if served and full else None,
+           "p99_ms": round(float(np.percentile(lat, 99)), 2) if served and full else None,
+           "p50_ms_served": round(float(np.percentile(lat, 50)), 2) if served else None,
+           "factored": st["factored"], "reused": st["reused"], "resident": st["resident"]}
+    print(f"[bench] serve: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def copy_probe(ch, n, nb, reps=3):
@@ -249,6 +361,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve", type=float, default=3.0, metavar="SECONDS",
+                    help="arrival window of the serving leg (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run an N x N device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -283,6 +398,8 @@ def main():
                 "avg_launch_ms": round(up["ms"] / up["launches"], 4) if up.get("launches") else None}
     slat = None if args.no_latency else solve_latency(ch, n, nb, 30)
     mixed = None if args.no_mixed else mixed_leg(ch, n, nrhs, nb, (1e2, 1e4, 1e6))
+    batch = None if args.no_batch else batch_leg(ch)
+    serve = serve_leg(ch, args.serve) if args.serve > 0 else None
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -310,6 +427,8 @@ def main():
                     for k, v in kst.items()},
         "solve_latency_ms": slat,
         "mixed": mixed,
+        "batch": batch,
+        "serve": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -299,10 +299,12 @@ class Server:
         self.keep = {}
 
     def submit(self, matrix_id, n, B, seed=0, A=None):
-        """B: Fortran float64 array (n x nrhs), overwritten by the solution."""
-        assert B.dtype == np.float64 and B.flags.f_contiguous
+        """B: float64 column-major array or column block of one (n x nrhs,
+        leading dimension from its strides), overwritten by the solution."""
+        assert B.dtype == np.float64 and B.strides[0] == 8 and B.shape[0] >= n
+        ldb = B.strides[1] // 8 if B.shape[1] > 1 else max(B.shape[0], 1)
         r = ServerRequest(matrix_id, n, B.shape[1], A.ctypes.data if A is not None else None,
-                          A.shape[0] if A is not None else 0, seed, B.ctypes.data, B.shape[0])
+                          A.shape[0] if A is not None else 0, seed, B.ctypes.data, ldb)
         t = ctypes.c_longlong()
         check(load().CHAMAMD_Server_Submit(self.ptr, ctypes.byref(r), ctypes.byref(t)), "CHAMAMD_Server_Submit")
         self.keep[t.value] = (A, B)

@@ -75,12 +75,27 @@ void potrf_run(const PotrfPlan &p)
             else k::trailing_update_lp(p.A, p.P[k & 1], p.lp_mode, NT, ldt, n0, n1, NT, p.d_info, s);
         });
     };
+    /* POTRF(k) runs next to the bulk update of step k-1 (columns k+1..).
+     * While that grid fills the device, a compute unit only frees one
+     * update workgroup's worth of registers at a time, which the 4-wave
+     * POTRF fits and the 8-wave one does not; once the bulk grid is smaller
+     * than the device, the faster 8-wave POTRF gets a compute unit of its
+     * own. */
+    int ncu = 0;
+    CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int bpt = std::max(1, (ldt / 128) * (ldt / 128));     /* update workgroups per tile */
+    auto potrf_waves = [&](int k) {
+        if (k == 0 || c->lookahead <= 0) return 8;
+        const long long S = NT - k - 1;                           /* columns of bulk update k-1 */
+        return (S * (S + 1) / 2) * bpt >= 2LL * ncu ? 4 : 8;
+    };
     auto panel = [&](int k) {
         const int nvalid = (k == NT - 1) ? p.nvalid_last : p.nb;
         double *dk = dinv + (size_t)k * dper;
         const double nk = tsz(k);
+        const int waves = potrf_waves(k);
         timed(CHAMAMD_KERNEL_POTRF, nk * (nk * nk / 3.0 + nk / 2.0 + 1.0 / 6.0), sm,
-              [&] { k::potrf_diag(tile(k, k), ldt, dk, p.nb * k, nvalid, p.d_info, sm); });
+              [&] { k::potrf_diag(tile(k, k), ldt, dk, p.nb * k, nvalid, p.d_info, waves, sm); });
         if (k + 1 < NT) {
             double f = 0.0;
             for (int m = k + 1; m < NT; ++m) f += tsz(m) * nbd * nbd;

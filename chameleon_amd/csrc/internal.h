@@ -177,7 +177,7 @@ void norm_partials(const double *d, int mt, int nt, int ldt, int wdt, int mb, in
 void lacpy_tiles(const double *src, double *dst, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N,
                  int uplo, hipStream_t s);
 /* chol.hip */
-void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, hipStream_t s);
+void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, int waves, hipStream_t s);
 void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int ldt, const double *Lkk,
                             const double *dinv, const int *info, hipStream_t s);
 void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, int ldt, const double *L0,

@@ -179,8 +179,6 @@ __device__ void assemble_inverse(const double (*Ld)[OBP], const double (*Lb)[16]
  * The strictly upper triangle of the matrix is never written.  A failed
  * job reports the 1-based column of its first non-positive pivot, only
  * for pivots inside its valid part (nvalid columns).                    */
-#define POTRF_THREADS 512
-#define POTRF_WAVES 8
 
 struct PotrfJobs {
     double *A;               /* job j: matrix at A + off[j] (off null: A)               */
@@ -198,14 +196,20 @@ struct PotrfJobs {
     int iinfo, batch;
 };
 
-__global__ __launch_bounds__(POTRF_THREADS) void potrf_jobs_kernel(PotrfJobs J)
+template <int POTRF_WAVES>
+__global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_kernel(PotrfJobs J)
 {
+    constexpr int POTRF_THREADS = POTRF_WAVES * 64;
+    constexpr int YH = 512 / POTRF_THREADS;     /* Y-slice columns staged per thread */
     __shared__ double Ld[OB][OBP];
-    __shared__ double Lv[OB][OBP];
+    /* Lv (the 64 x 64 inverse, phases (c)-(d)) and Ys (the staged Y slices
+     * of phase (a), 2 x 64 rows x 16 k) are never live at the same time */
+    __shared__ double LvYs[OB * OBP];
     __shared__ double Lb[4][16][17];
     __shared__ double Rd[16];
-    __shared__ double Ys[2][16 * OB];   /* Y slice of phase (a): 64 rows x 16 k, staged per slice */
     __shared__ int s_fail, s_job;
+    double (*Lv)[OBP] = reinterpret_cast<double (*)[OBP]>(LvYs);
+    double (*Ys)[16 * OB] = reinterpret_cast<double (*)[16 * OB]>(LvYs);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     if (!J.batch && *J.info != 0) return;
@@ -240,22 +244,33 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_jobs_kernel(PotrfJobs J)
                 v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
                 const double *px = T + (r0 + li) + (size_t)lk * ldt;
                 /* Y slice element (row r < 64, k q < 16) -> Ys[q * 64 + r];
-                 * thread tid moves rows 2*(tid%32)..+1 of column tid/32 */
+                 * thread tid moves rows 2*(tid%32)..+1 of columns
+                 * tid/32 + h * (threads/32), h < YH */
                 const int yq = tid >> 5, yr = (tid & 31) * 2;
                 const bool yv = yr < W;
-                v2d yreg = {0.0, 0.0};
-                if (yv) yreg = *reinterpret_cast<const v2d *>(T + (C + yr) + (size_t)yq * ldt);
+                v2d yreg[YH];
+#pragma unroll
+                for (int h = 0; h < YH; ++h) {
+                    yreg[h] = v2d{0.0, 0.0};
+                    if (yv) yreg[h] = *reinterpret_cast<const v2d *>(T + (C + yr) + (size_t)(yq + h * (POTRF_THREADS / 32)) * ldt);
+                }
                 double x[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) x[q] = active ? px[(size_t)(4 * q) * ldt] : 0.0;
-                *reinterpret_cast<v2d *>(&Ys[0][yq * OB + yr]) = yreg;
+#pragma unroll
+                for (int h = 0; h < YH; ++h)
+                    *reinterpret_cast<v2d *>(&Ys[0][(yq + h * (POTRF_THREADS / 32)) * OB + yr]) = yreg[h];
                 __syncthreads();
                 for (int p = 0; p < C; p += 16) {
                     const int cur = (p >> 4) & 1;
                     const bool more = p + 16 < C;
                     double xn[4];
                     if (more) {
-                        if (yv) yreg = *reinterpret_cast<const v2d *>(T + (C + yr) + (size_t)(p + 16 + yq) * ldt);
+#pragma unroll
+                        for (int h = 0; h < YH; ++h)
+                            if (yv)
+                                yreg[h] = *reinterpret_cast<const v2d *>(
+                                    T + (C + yr) + (size_t)(p + 16 + yq + h * (POTRF_THREADS / 32)) * ldt);
 #pragma unroll
                         for (int q = 0; q < 4; ++q) xn[q] = active ? px[(size_t)(p + 16 + 4 * q) * ldt] : 0.0;
                     }
@@ -267,7 +282,9 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_jobs_kernel(PotrfJobs J)
                                 if (f < nfb) acc[f] = mfma_f64(Ys[cur][(4 * q + lk) * OB + 16 * f + li], x[q], acc[f]);
                     }
                     if (more) {
-                        *reinterpret_cast<v2d *>(&Ys[cur ^ 1][yq * OB + yr]) = yreg;
+#pragma unroll
+                        for (int h = 0; h < YH; ++h)
+                            *reinterpret_cast<v2d *>(&Ys[cur ^ 1][(yq + h * (POTRF_THREADS / 32)) * OB + yr]) = yreg[h];
 #pragma unroll
                         for (int q = 0; q < 4; ++q) x[q] = xn[q];
                     }
@@ -398,13 +415,14 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_jobs_kernel(PotrfJobs J)
   }
 }
 
-void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, hipStream_t s)
+void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, int waves, hipStream_t s)
 {
     PotrfJobs J = {};
     J.A = Akk; J.dinv = dinv;
     J.np0 = ldt; J.ld0 = ldt; J.nvalid0 = nvalid;
     J.count = 1; J.info = info; J.iinfo = iinfo; J.batch = 0;
-    hipLaunchKernelGGL(potrf_jobs_kernel, dim3(1), dim3(POTRF_THREADS), 0, s, J);
+    if (waves == 8) hipLaunchKernelGGL(potrf_jobs_kernel<8>, dim3(1), dim3(512), 0, s, J);
+    else hipLaunchKernelGGL(potrf_jobs_kernel<4>, dim3(1), dim3(256), 0, s, J);
 }
 
 /* Batched form: `count` matrices, job j of order np[j] (multiple of 16)
@@ -426,13 +444,13 @@ void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, c
             int dev = 0, ncu = 0, per = 0;
             CHX_HIP(hipGetDevice(&dev));
             CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, potrf_jobs_kernel, POTRF_THREADS, 0));
+            CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, potrf_jobs_kernel<4>, 256, 0));
             resident = ncu * (per > 0 ? per : 1);
         }
         grid = resident;
     }
     if (grid > count) grid = count;
-    hipLaunchKernelGGL(potrf_jobs_kernel, dim3(grid), dim3(POTRF_THREADS), 0, s, J);
+    hipLaunchKernelGGL(potrf_jobs_kernel<4>, dim3(grid), dim3(256), 0, s, J);
 }
 
 /* ================================================================== */

@@ -15,7 +15,8 @@
  * that no pending request needs.  A request whose factor cannot fit even
  * in an empty pool is rejected (CHAMELEON_ERR_OUT_OF_RESOURCES).
  *
- * A round (CHAMAMD_Server_Progress) takes every pending request:
+ * A round (CHAMAMD_Server_Progress) takes the oldest pending requests (at
+ * most CHAMAMD_SERVE_ROUND, default 256):
  *   1. matrices not resident are placed and generated (or uploaded);
  *   2. systems of order <= CHAMAMD_SERVE_SMALL are factored together by
  *      the job-queue POTRF kernel (one workgroup per system, largest
@@ -118,6 +119,7 @@ struct CHAMAMD_server_s {
     char *base = nullptr;
     int small_max = 1024;
     int large_nb = 512;
+    int max_round = 256;          /* requests per round (CHAMAMD_SERVE_ROUND) */
     long long next_ticket = 1;
     std::deque<Req> pending;
     std::unordered_map<long long, Factor *> factors;
@@ -224,8 +226,10 @@ int round(CHAMAMD_server_t *s)
 {
     Context *c = ctx();
     hipStream_t sm = c->s_main;
-    std::vector<Req> reqs(s->pending.begin(), s->pending.end());
-    s->pending.clear();
+    /* continuous batching: the oldest requests, at most max_round of them */
+    const size_t take = std::min(s->pending.size(), (size_t)s->max_round);
+    std::vector<Req> reqs(s->pending.begin(), s->pending.begin() + take);
+    s->pending.erase(s->pending.begin(), s->pending.begin() + take);
     if (reqs.empty()) return 0;
 
     /* 1. residency */
@@ -494,6 +498,8 @@ int CHAMAMD_Server_Create(CHAMAMD_server_t **server, size_t hbm_budget_bytes)
     s->pool.init(hbm_budget_bytes);
     const char *v = getenv("CHAMAMD_SERVE_SMALL");
     if (v && *v) s->small_max = std::max(16, atoi(v));
+    v = getenv("CHAMAMD_SERVE_ROUND");
+    if (v && *v) s->max_round = std::max(1, atoi(v));
     s->large_nb = ctx()->nb;
     *server = s;
     return CHAMELEON_SUCCESS;

@@ -107,7 +107,7 @@ int CHAMAMD_Batch_Info(CHAMAMD_batch_t *batch, int *info);
 /* Solve server: requests A X = B arrive over time; factors stay resident
  * in a device pool of a fixed HBM budget (least recently used factors not
  * needed by pending requests are evicted), and each Progress call serves
- * everything pending as one batched round.  No counterpart in the
+ * the oldest pending requests as one batched round.  No counterpart in the
  * reference beyond its sequence/request interface
  * (include/chameleon/runtime_struct.h:78-103), which has no admission
  * control, residency or latency accounting.  Host-thread model as the
@@ -131,7 +131,8 @@ typedef struct {
 int CHAMAMD_Server_Create(CHAMAMD_server_t **server, size_t hbm_budget_bytes);
 int CHAMAMD_Server_Destroy(CHAMAMD_server_t **server);
 int CHAMAMD_Server_Submit(CHAMAMD_server_t *server, const CHAMAMD_request_t *req, long long *ticket);
-/* serve everything pending (blocking); *processed = requests taken */
+/* serve one round of the oldest pending requests (blocking; at most
+ * CHAMAMD_SERVE_ROUND of them, default 256); *processed = requests taken */
 int CHAMAMD_Server_Progress(CHAMAMD_server_t *server, int *processed);
 /* *done = 1 once the request finished; *status = 0, the not-positive-
  * definite column, or a negative error (CHAMELEON_ERR_OUT_OF_RESOURCES

@@ -127,7 +127,7 @@ def test_tile_batch_potrf_reports_failing_tile(ch):
     M, N, nb = 128, 64, 32
     say("tile batch failure")
     A = oracle.dplgsy_batch(float(nb), M, N, nb, nb, 7)
-    A[2 * nb + 5, 2 * nb + 5] = -1.0                       # tile (2, 0): fails at column 6
+    A[2 * nb + 5, 5] = -1.0                                # tile (2, 0): fails at column 6
     Ar = A.copy(order="F")
     with ch.Desc.from_array(A, nb) as dA:
         rc = ch.lib.CHAMELEON_dpotrf_batch_Tile(L_, dA.ptr)
