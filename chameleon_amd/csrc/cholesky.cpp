@@ -153,9 +153,8 @@ void potrf_run(const PotrfPlan &p)
 }
 
 /* Workspace of the solves.  With ldt a multiple of 64 the strip solve runs
- * (kernels/solve.hip): 64-row strips use the 64 x 64 inverses of the
- * diagonal blocks (dinv), 128-row strips the transposed inverses of whole
- * diagonal tiles (TT).  Other tile sizes use the tile-step cooperative solve, which
+ * (kernels/solve.hip) with the 64 x 64 inverses of the diagonal blocks
+ * (dinv).  Other tile sizes use the tile-step cooperative solve, which
  * needs the tile inverses in both orientations (TT = L_kk^-T, TN = L_kk^-1)
  * and one solved block W.  BT holds the transposed right-hand sides
  * (nrp x NT*ldt). */
@@ -167,22 +166,22 @@ SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info)
     const size_t ts = (size_t)ldt * ldt;
     const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
     SolvePlan p;
-    p.R = (ldt % 64 != 0) ? 0 : (c->solve_strip_rows == 128 && ldt % 128 == 0) ? 128 : 64;
-    const size_t ntile_inv = p.R == 64 ? 0 : p.R == 128 ? 1 : 2;
+    p.R = (ldt % 64 == 0) ? 64 : 0;
+    const size_t ntile_inv = p.R == 64 ? 0 : 2;
     const size_t need = (ntile_inv * NT * ts + (size_t)nrp * NT * ldt + (size_t)nrp * ldt + (size_t)NT * dper) *
                         sizeof(double);
     double *ws = (double *)workspace(need);
     p.BT = ws;
     p.W = p.BT + (size_t)nrp * NT * ldt;
     p.dinv = p.W + (size_t)nrp * ldt;
-    p.TT = p.dinv + (size_t)NT * dper;
-    p.TN = ntile_inv == 2 ? p.TT + (size_t)NT * ts : nullptr;
+    p.TT = ntile_inv ? p.dinv + (size_t)NT * dper : nullptr;
+    p.TN = ntile_inv ? p.TT + (size_t)NT * ts : nullptr;
     k::dinv_all(A, NT, (size_t)(NT + 1) * ts, ldt, p.dinv, sm);
     if (p.R == 64) return p;
     k::set_identity(p.TT, NT, ldt, sm);
     k::trsm_right_lower_trans_batched(p.TT, NT, (long long)ts, ldt, A, (long long)(NT + 1) * ts, p.dinv,
                                       (long long)dper, d_info, sm);
-    if (p.R == 0) k::tile_transpose(p.TT, p.TN, NT, ldt, sm);
+    k::tile_transpose(p.TT, p.TN, NT, ldt, sm);
     return p;
 }
 
@@ -196,25 +195,15 @@ static void solve_pair(const SolvePlan &p, double *A, int NT, int ldt, double *B
         k::solve_coop(A, p.TN, p.TT, BT, p.W, NT, ldt, nrp, 1, d_info, sm);
         return;
     }
-    const int S = NT * ldt / p.R;
+    const int S = NT * ldt / 64;
     const int chunks = (nrp + 63) / 64;
     int epoch = 0;
     int *flags = strip_flags(S, 2 * chunks, &epoch);
-    const size_t ts = (size_t)ldt * ldt;
     const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
-    if (p.R == 64) {
-        /* 64 x 64 inverse of block b of diagonal tile t at dinv + (t*nob + b)*4096, ld 64 */
-        for (int dir = 0; dir < 2; ++dir)
-            k::solve_strips(A, p.dinv, (long long)dper, 64LL * 64, 64, 0, 64, BT, NT, ldt, nrp, dir, flags,
-                            epoch + dir * chunks, d_info, sm);
-    } else {
-        /* TT: the tile inverses transposed, L_tt^-T column-major with ld ldt;
-         * the inverse of a diagonal R x R block of L_tt is the matching
-         * block of L_tt^-1 */
-        for (int dir = 0; dir < 2; ++dir)
-            k::solve_strips(A, p.TT, (long long)ts, (long long)p.R * (ldt + 1), ldt, 1, p.R, BT, NT, ldt, nrp, dir,
-                            flags, epoch + dir * chunks, d_info, sm);
-    }
+    /* 64 x 64 inverse of block b of diagonal tile t at dinv + (t*nob + b)*4096, ld 64 */
+    for (int dir = 0; dir < 2; ++dir)
+        k::solve_strips(A, p.dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
+                        d_info, sm);
 }
 
 /* In-place solve of transposed right-hand sides BT (nrp x NT*ldt) with the

@@ -76,7 +76,6 @@ struct Context {
     int *strip_flags = nullptr;
     int strip_flags_n = 0;
     int strip_epoch = 0;
-    int solve_strip_rows = 64;    /* 64 or 128 (CHAMAMD_SOLVE_STRIP) */
     int sched = 0;                /* lookahead issue order (CHAMAMD_SCHED) */
     long long st_launches[4] = {0, 0, 0, 0};
     double st_ms[4] = {0, 0, 0, 0};
@@ -144,7 +143,7 @@ struct PotrfPlan {
 void potrf_run(const PotrfPlan &p);
 struct SolvePlan {
     double *TT, *TN, *BT, *W, *dinv;   /* diagonal-tile inverses, transposed RHS, one block, 64x64 inverses */
-    int R = 0;                         /* strip rows of the barrier-free solve; 0: tile-step solve */
+    int R = 0;                         /* 64: strip solve; 0: tile-step solve */
 };
 SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info);
 void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nrp, int *d_info);
@@ -200,8 +199,9 @@ void tile_pad_diag(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, 
 void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int mode, hipStream_t s);
 void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, int n0, int n1, int NT, const int *info,
                         hipStream_t s);
-void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp,
+void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s);
+size_t residual_scratch(int NT, int ldt, int nrp);   /* doubles of `part` */
 void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStream_t s);
 void axpy(double *Y, const double *X, size_t n, hipStream_t s);
 /* cond.hip */
@@ -209,8 +209,8 @@ void cond_init(double *A, int NT, int ldt, int N, double kappa, hipStream_t s);
 void cond_gemv(const double *A, int NT, int ldt, int N, const double *v, double *w, hipStream_t s);
 void cond_rank2(double *A, int NT, int ldt, int N, const double *v, const double *z, hipStream_t s);
 /* solve.hip */
-int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, int dtrans, int R,
-                 double *BT, int NT, int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s);
+int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
+                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s);
 void solve_coop(const double *A, const double *TN, const double *TT, double *BT, double *W, int NT, int ldt, int nrhs,
                 int trans, const int *info, hipStream_t s);
 void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s);

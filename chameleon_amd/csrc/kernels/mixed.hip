@@ -302,61 +302,152 @@ void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, in
 /* ------------------------------------------------------------------ */
 /* FP64 residual R = B - A X for the symmetric A whose lower triangle is the
  * tile mirror A (NT x NT tiles of ldt x ldt), right-hand sides transposed
- * (element (j, p) at j + p*nrp).  One wavefront per 16 rows x 64 columns;
- * blocks below the diagonal tile are read as stored, blocks above as the
- * transpose of the stored lower tile, the diagonal tile by triangle. */
-__global__ __launch_bounds__(256) void residual_kernel(const double *A, int NT, int ldt, const double *XT,
-                                                        const double *BT, double *RT, int nrp)
+ * (element (j, p) at j + p*nrp).
+ *
+ * The row range of A is cut in RES_KS pieces: wavefront (rb, jc, ks)
+ * forms the 16 x 64 block (rows 16 rb.., columns 64 jc..) of A(:, piece
+ * ks) X(piece ks, :) into its own partial buffer, and a second kernel adds
+ * the pieces in a fixed order (deterministic).  Blocks below the diagonal
+ * tile are read as stored, blocks above as the transpose of the stored
+ * lower tile (32-byte row vectors), the diagonal tile by triangle.  The
+ * k loop runs in 16-deep chunks with the next chunk's loads in flight. */
+#define RES_KS 8
+__global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int NT, int ldt, const double *XT, int nrp,
+                                                        double *part)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int rows = NT * ldt;
     const int jchunks = (nrp + 63) / 64;
-    const int rb = gw / jchunks, jc = gw % jchunks;
-    if (rb * 16 >= rows) return;
+    const int nrb = rows / 16;
+    const int ks = gw % RES_KS, rest = gw / RES_KS;
+    const int rb = rest % nrb, jc = rest / nrb;
+    if (jc >= jchunks) return;
     const int r0 = rb * 16;
     const int m = r0 / ldt, i0 = r0 % ldt;
     const int j0 = jc * 64;
     const int nf = min(4, (nrp - j0) / 16);
     const size_t ts = (size_t)ldt * ldt;
+    /* this wave's k range: [kb, ke) in units of 16 */
+    const int nk16 = rows / 16;
+    const int kb = (int)((long long)nk16 * ks / RES_KS) * 16, ke = (int)((long long)nk16 * (ks + 1) / RES_KS) * 16;
     v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
-    for (int nt = 0; nt < NT; ++nt) {
-        const double *T;
-        long long si, sp;   /* opA(i, p) at T[i*si + p*sp] */
-        if (nt < m) { T = A + ((size_t)nt * NT + m) * ts + i0; si = 1; sp = ldt; }
-        else if (nt > m) { T = A + ((size_t)m * NT + nt) * ts + (size_t)i0 * ldt; si = ldt; sp = 1; }
-        else { T = A + ((size_t)m * NT + m) * ts; si = 0; sp = 0; }
-        for (int p = 0; p < ldt; p += 4) {
-            double a;
-            if (nt != m) {
-                a = T[(long long)li * si + (long long)(p + lk) * sp];
-            } else {
-                const int ii = i0 + li, pp = p + lk;
-                a = (ii >= pp) ? T[ii + (size_t)pp * ldt] : T[pp + (size_t)ii * ldt];
-            }
-            const double *xr = XT + (size_t)(nt * ldt + p + lk) * nrp + j0 + li;
+    /* x operands of chunk p0: X(p0 + 4 lk + q, j0 + 16 f + li) */
+    auto load_x = [&](int p0, double (&x)[4][4]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double *xr = XT + (size_t)(p0 + 4 * lk + q) * nrp + j0 + li;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) x[q][f] = (f < nf) ? xr[16 * f] : 0.0;
+        }
+    };
+    auto mma = [&](const double (&a)[4], const double (&x)[4][4]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int f = 0; f < 4; ++f)
-                if (f < nf) acc[f] = mfma_f64(a, xr[16 * f], acc[f]);
+                if (f < nf) acc[f] = mfma_f64(a[q], x[q][f], acc[f]);
+    };
+    /* the k range, tile by tile: three loop shapes, each software-pipelined */
+    for (int nt = kb / ldt; nt * ldt < ke; ++nt) {
+        const int pb = max(kb, nt * ldt) - nt * ldt, pe = min(ke, (nt + 1) * ldt) - nt * ldt;
+        if (nt < m) {
+            /* stored tile (m, nt): element (i0 + li, p) at T[i0 + li + p*ldt] */
+            const double *T = A + ((size_t)nt * NT + m) * ts + i0 + li;
+            double a[4], x[4][4], an[4], xn[4][4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = T[(size_t)(pb + 4 * lk + q) * ldt];
+            load_x(nt * ldt + pb, x);
+            for (int p = pb; p < pe; p += 16) {
+                const bool more = p + 16 < pe;
+                if (more) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) an[q] = T[(size_t)(p + 16 + 4 * lk + q) * ldt];
+                    load_x(nt * ldt + p + 16, xn);
+                }
+                mma(a, x);
+                if (more) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        a[q] = an[q];
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
+                    }
+                }
+            }
+        } else if (nt > m) {
+            /* transpose of stored tile (nt, m): element (i0 + li, p) at T[p + (i0 + li)*ldt] */
+            const double *T = A + ((size_t)m * NT + nt) * ts + (size_t)(i0 + li) * ldt;
+            double a[4], x[4][4], an[4], xn[4][4];
+            {
+                const v4d v = *(const v4d *)(T + pb + 4 * lk);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[q] = v[q];
+            }
+            load_x(nt * ldt + pb, x);
+            for (int p = pb; p < pe; p += 16) {
+                const bool more = p + 16 < pe;
+                if (more) {
+                    const v4d v = *(const v4d *)(T + p + 16 + 4 * lk);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) an[q] = v[q];
+                    load_x(nt * ldt + p + 16, xn);
+                }
+                mma(a, x);
+                if (more) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        a[q] = an[q];
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
+                    }
+                }
+            }
+        } else {
+            /* diagonal tile, lower triangle stored */
+            const double *T = A + ((size_t)m * NT + m) * ts;
+            const int ii = i0 + li;
+            for (int p = pb; p < pe; p += 16) {
+                double a[4], x[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int pp = p + 4 * lk + q;
+                    a[q] = T[(ii >= pp) ? ii + (size_t)pp * ldt : pp + (size_t)ii * ldt];
+                }
+                load_x(nt * ldt + p, x);
+                mma(a, x);
+            }
         }
     }
-    /* slot s of acc[f]: row i0 + lk + 4s, column j0 + 16 f + li */
+    /* slot s of acc[f]: row r0 + lk + 4s, column j0 + 16 f + li */
+    double *P = part + (size_t)ks * rows * nrp;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
         if (f < nf) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const size_t e = (j0 + 16 * f + li) + (size_t)(r0 + lk + 4 * s) * nrp;
-                RT[e] = BT[e] - acc[f][s];
-            }
+            for (int s = 0; s < 4; ++s) P[(j0 + 16 * f + li) + (size_t)(r0 + lk + 4 * s) * nrp] = acc[f][s];
         }
 }
 
-void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp,
+__global__ void residual_sum_kernel(const double *BT, const double *part, double *RT, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < RES_KS; ++k) s += part[(size_t)k * n + e];
+        RT[e] = BT[e] - s;
+    }
+}
+
+size_t residual_scratch(int NT, int ldt, int nrp) { return (size_t)RES_KS * NT * ldt * nrp; }
+
+void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s)
 {
-    const int waves = (NT * ldt / 16) * ((nrp + 63) / 64);
-    hipLaunchKernelGGL(residual_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, A, NT, ldt, XT, BT, RT, nrp);
+    const long long waves = (long long)(NT * ldt / 16) * ((nrp + 63) / 64) * RES_KS;
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
+    const size_t n = (size_t)NT * ldt * nrp;
+    hipLaunchKernelGGL(residual_sum_kernel, dim3(2048), dim3(256), 0, s, BT, part, RT, n);
 }
 
 /* per-column max |V(j, p)| over p < nrows (V transposed: j + p*nrp); out

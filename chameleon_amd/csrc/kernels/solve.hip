@@ -156,21 +156,23 @@ void solve_coop(const double *A, const double *TN, const double *TT, double *BT,
 /* ------------------------------------------------------------------ */
 /* Strip solve without grid-wide barriers.
  *
- * The rows are cut in strips of R rows (R = 64 or 128, dividing ldt).
- * Strip s is owned by one workgroup of R/16 wavefronts, wave w holding
- * rows 16w .. 16w+15 of the strip and all NC*16 columns of this launch's
- * right-hand-side chunk.  The workgroup keeps acc = B_s in registers,
- * subtracts the contribution of every strip on the near side of it as soon
- * as that strip's solution is published, then applies the inverse of its
- * diagonal R x R block and publishes X_s:
+ * The rows are cut in strips of 64 (64 divides ldt).  Strip s is owned by
+ * one workgroup of 4 wavefronts, wave w holding rows 16w .. 16w+15 of the
+ * strip and all NC*16 columns of this launch's right-hand-side chunk.  The
+ * workgroup keeps acc = B_s in registers, subtracts the contribution of
+ * every strip on the near side of it as soon as that strip's solution is
+ * published, then applies the inverse of its 64 x 64 diagonal block and
+ * publishes X_s:
  *   forward  (L Y = B):    acc -= L(s, j) X_j     for j = 0 .. s-1
  *                          X_s  = D_s acc           D_s = L(s,s)^-1
  *   backward (L^T X = Y):  acc -= L(j, s)^T X_j   for j = S-1 .. s+1
  *                          X_s  = D_s^T acc
- * The strips form a chain (s needs s-1): the time per link bounds the
- * solve, so everything of a link that does not depend on the previous
- * strip -- the rows of D_s and the factor block against the previous
- * strip -- is loaded into registers before the wait.
+ * A strip far down the chain consumes hundreds of published blocks, so
+ * the block loop is a pipeline: while block j is multiplied, the factor
+ * rows of block j+1 are loaded into registers (before waiting for its
+ * publication: the factor does not depend on it) and its solution block
+ * into staging registers, which go to LDS (one copy for the 4 waves)
+ * after the block is done.  D_s goes to LDS before any wait.
  *
  * Publication is a per-strip flag set to this launch's epoch after a
  * device-scope release; readers poll the flags and acquire.  A flag of
@@ -186,11 +188,10 @@ void solve_coop(const double *A, const double *TN, const double *TT, double *BT,
  * vector loads.                                                          */
 struct StripArgs {
     const double *A;      /* factor mirror (Lower orientation)                       */
-    const double *D;      /* inverses of the diagonal R x R blocks, element (r, p) of
+    const double *D;      /* 64 x 64 inverses of the diagonal blocks, element (r, p) of
                            * strip s at D[(s/q)*dts + (s%q)*doff + r + p*ldd]         */
     long long dts, doff;
     int ldd;
-    int dtrans;           /* D holds the transposes L(s,s)^-T instead                 */
     double *BT;           /* right-hand sides, element (c, r) at c + r * nrp         */
     int *flags;           /* one per strip                                            */
     int NT, ldt, nrp, c0; /* c0: first column of this launch's chunk                  */
@@ -215,60 +216,46 @@ __device__ __forceinline__ void load_op4(const double *M, int ld, bool vec, int 
     }
 }
 
-template <int R, int NC>
-__global__ __launch_bounds__(R * 4) void strip_solve_kernel(StripArgs a)
+#define SR 64            /* strip rows */
+template <int NC>
+__global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
 {
-    constexpr int NCOL = 16 * NC, NPC = R / 16;
-    __shared__ double Xs[R * NCOL];     /* acc of the strip, input of the diagonal product */
-    __shared__ double Xc[R * NCOL];     /* solution of the chain predecessor               */
+    constexpr int NCOL = 16 * NC, NPC = SR / 16;
+    constexpr int XV = SR * NCOL / (2 * 256);     /* 16-byte pieces of a block staged per thread */
+    /* the solution block being consumed; after the block loop, the acc of
+     * the strip (input of the diagonal product) */
+    __shared__ double Xb[SR * NCOL];
+    __shared__ double Ds[SR * (SR + 1)];          /* D_s, element (r, p) at r + p*65 */
+    double *Xs = Xb;
     if (*a.info != 0) return;     /* uniform over the grid: nobody is left waiting */
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const int q = a.ldt / R, nrp = a.nrp, ldt = a.ldt;
+    const int q = a.ldt / SR, nrp = a.nrp, ldt = a.ldt;
     const size_t ts = (size_t)ldt * ldt;
     const int wr = wave * 16;
     const bool fwd = a.trans == 0;
-    /* operand addressing of the factor blocks: forward reads L(s, j)
-     * column-wise, backward reads L(j, s)^T as row vectors */
-    const bool lvec = !fwd;
-    const bool dvec = fwd ? a.dtrans != 0 : a.dtrans == 0;
+    const bool lvec = !fwd;       /* backward reads L(j, s)^T as row vectors */
     for (int t = blockIdx.x; t < a.S; t += gridDim.x) {
         const int s = fwd ? t : a.S - 1 - t;
-        const int tm = s / q, rs = (s % q) * R;
-        double *Bs = a.BT + a.c0 + (size_t)s * R * nrp;
+        const int tm = s / q, rs = (s % q) * SR;
+        double *Bs = a.BT + a.c0 + (size_t)s * SR * nrp;
         auto block_of = [&](int j) -> const double * {
-            const int tj = j / q, rj = (j % q) * R;
+            const int tj = j / q, rj = (j % q) * SR;
             return fwd ? a.A + ((size_t)tj * a.NT + tm) * ts + rs + (size_t)rj * ldt
                        : a.A + ((size_t)tm * a.NT + tj) * ts + rj + (size_t)rs * ldt;
         };
+        auto xblock = [&](int j) -> const double * { return a.BT + a.c0 + (size_t)j * SR * nrp; };
 
-        /* registers filled before any wait: the right-hand side, this
-         * wave's rows of D_s, and the factor block against the chain
-         * predecessor */
         v4d acc[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[c][e] = Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp];
+        /* D_s into LDS before any wait (read by the diagonal product) */
         const double *Dg = a.D + (size_t)tm * a.dts + (size_t)(s % q) * a.doff;
+#pragma unroll 4
+        for (int e = tid; e < SR * SR; e += 256) Ds[(e % SR) + (e / SR) * (SR + 1)] = Dg[(e % SR) + (size_t)(e / SR) * a.ldd];
         const int pc0 = fwd ? 0 : wave, pc1 = fwd ? wave + 1 : NPC;   /* nonzero chunks of row group `wave` */
-        double dreg[NPC][4];
-#pragma unroll
-        for (int pc = 0; pc < NPC; ++pc) {
-            if (pc >= pc0 && pc < pc1) load_op4(Dg, a.ldd, dvec, wr, pc * 16, li, lk, dreg[pc]);
-            else {
-#pragma unroll
-                for (int qq = 0; qq < 4; ++qq) dreg[pc][qq] = 0.0;
-            }
-        }
-        const int jc = fwd ? s - 1 : s + 1;          /* chain predecessor */
-        const bool has_jc = fwd ? s > 0 : s < a.S - 1;
-        double lreg[NPC][4];
-        if (has_jc) {
-            const double *L = block_of(jc);
-#pragma unroll
-            for (int pc = 0; pc < NPC; ++pc) load_op4(L, ldt, lvec, wr, pc * 16, li, lk, lreg[pc]);
-        }
 
         int ready = fwd ? -1 : a.S;
         auto wait_for = [&](int j) {
@@ -291,44 +278,67 @@ __global__ __launch_bounds__(R * 4) void strip_solve_kernel(StripArgs a)
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         };
+        auto stage_load = [&](int j, v2d (&xv)[XV]) {
+            const double *Xj = xblock(j);
+#pragma unroll
+            for (int h = 0; h < XV; ++h) {
+                const int e = 2 * (tid + 256 * h), r = e / NCOL, c = e % NCOL;
+                xv[h] = *(const v2d *)&Xj[c + (size_t)r * nrp];
+            }
+        };
+        auto stage_store = [&](const v2d (&xv)[XV]) {
+#pragma unroll
+            for (int h = 0; h < XV; ++h) *(v2d *)&Xb[2 * (tid + 256 * h)] = xv[h];
+        };
 
-        /* strips before the chain predecessor */
-        const int jstep = fwd ? 1 : -1;
-        for (int j = fwd ? 0 : a.S - 1; j != jc && has_jc; j += jstep) {
-            wait_for(j);
-            const double *Xj = a.BT + a.c0 + (size_t)j * R * nrp;
-            const double *L = block_of(j);
-#pragma unroll 2
-            for (int pc = 0; pc < NPC; ++pc) {
-                double av[4], bv[NC][4];
-                load_op4(L, ldt, lvec, wr, pc * 16, li, lk, av);
+        /* blocks j in chain order; the last one is the chain predecessor.
+         * Block j is multiplied before the wait for block j+1, so a strip
+         * at the front of the chain has its partial sum ready when its
+         * predecessor publishes. */
+        const int nblk = fwd ? s : a.S - 1 - s;
+        if (nblk > 0) {
+            double lcur[NPC][4], lnext[NPC][4];
+            v2d xv[XV];
+            int j = fwd ? 0 : a.S - 1;
+            {
+                const double *L = block_of(j);
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) bv[c][qq] = Xj[(c * 16 + li) + (size_t)(pc * 16 + 4 * lk + qq) * nrp];
-#pragma unroll
-                for (int qq = 0; qq < 4; ++qq)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) acc[c] = mfma_f64(-av[qq], bv[c][qq], acc[c]);
+                for (int pc = 0; pc < NPC; ++pc) load_op4(L, ldt, lvec, wr, pc * 16, li, lk, lcur[pc]);
+                wait_for(j);
+                stage_load(j, xv);
+                stage_store(xv);
+                __syncthreads();
             }
-        }
-        /* the chain predecessor: its factor block is already in registers,
-         * its solution is staged once in LDS for all waves */
-        if (has_jc) {
-            wait_for(jc);
-            const double *Xj = a.BT + a.c0 + (size_t)jc * R * nrp;
-            for (int e = threadIdx.x; e < R * NCOL / 2; e += R * 4) {
-                const int r = (2 * e) / NCOL, c = (2 * e) % NCOL;
-                *(v2d *)&Xc[r * NCOL + c] = *(const v2d *)&Xj[c + (size_t)r * nrp];
+            for (int b = 0; b < nblk; ++b) {
+                const bool more = b + 1 < nblk;
+                const int jn = fwd ? j + 1 : j - 1;
+                if (more) {
+                    const double *L = block_of(jn);
+#pragma unroll
+                    for (int pc = 0; pc < NPC; ++pc) load_op4(L, ldt, lvec, wr, pc * 16, li, lk, lnext[pc]);
+                }
+#pragma unroll
+                for (int pc = 0; pc < NPC; ++pc)
+#pragma unroll
+                    for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                        for (int c = 0; c < NC; ++c)
+                            acc[c] = mfma_f64(-lcur[pc][qq], Xb[(pc * 16 + 4 * lk + qq) * NCOL + c * 16 + li], acc[c]);
+                if (more) {
+                    wait_for(jn);
+                    stage_load(jn, xv);
+                }
+                __syncthreads();                 /* every wave is done with Xb */
+                if (more) {
+                    stage_store(xv);
+#pragma unroll
+                    for (int pc = 0; pc < NPC; ++pc)
+#pragma unroll
+                        for (int qq = 0; qq < 4; ++qq) lcur[pc][qq] = lnext[pc][qq];
+                }
+                __syncthreads();                 /* the next block is in Xb */
+                j = jn;
             }
-            __syncthreads();
-#pragma unroll
-            for (int pc = 0; pc < NPC; ++pc)
-#pragma unroll
-                for (int qq = 0; qq < 4; ++qq)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c)
-                        acc[c] = mfma_f64(-lreg[pc][qq], Xc[(pc * 16 + 4 * lk + qq) * NCOL + c * 16 + li], acc[c]);
         }
 
         /* diagonal block: X_s = D_s acc (forward) or D_s^T acc (backward) */
@@ -340,26 +350,30 @@ __global__ __launch_bounds__(R * 4) void strip_solve_kernel(StripArgs a)
         v4d x[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) x[c] = zero4d();
+        for (int pc = pc0; pc < pc1; ++pc) {
 #pragma unroll
-        for (int pc = 0; pc < NPC; ++pc) {
-            if (pc < pc0 || pc >= pc1) continue;
+            for (int qq = 0; qq < 4; ++qq) {
+                const int p = pc * 16 + 4 * lk + qq, r = wr + li;
+                const double dv = fwd ? Ds[r + p * (SR + 1)] : Ds[p + r * (SR + 1)];
 #pragma unroll
-            for (int qq = 0; qq < 4; ++qq)
-#pragma unroll
-                for (int c = 0; c < NC; ++c)
-                    x[c] = mfma_f64(dreg[pc][qq], Xs[(pc * 16 + 4 * lk + qq) * NCOL + c * 16 + li], x[c]);
+                for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv, Xs[p * NCOL + c * 16 + li], x[c]);
+            }
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int e = 0; e < 4; ++e) Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp] = x[c][e];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        /* publication: the barrier completes every wave's stores, then one
+         * device-scope release by one thread covers them all */
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(&a.flags[s], a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&a.flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
-template <int R, int NC>
+template <int NC>
 static void strip_launch(const StripArgs &a, hipStream_t s)
 {
     static int per_cu = -1, ncu = 0;
@@ -367,24 +381,24 @@ static void strip_launch(const StripArgs &a, hipStream_t s)
         int dev = 0;
         CHX_HIP(hipGetDevice(&dev));
         CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, strip_solve_kernel<R, NC>, R * 4, 0));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, strip_solve_kernel<NC>, 256, 0));
         if (per_cu < 1) per_cu = 1;
     }
     int grid = ncu * per_cu;
     if (grid > a.S) grid = a.S;
     StripArgs x = a;
     void *args[] = {&x};
-    CHX_HIP(hipLaunchCooperativeKernel((const void *)strip_solve_kernel<R, NC>, dim3(grid), dim3(R * 4), args, 0, s));
+    CHX_HIP(hipLaunchCooperativeKernel((const void *)strip_solve_kernel<NC>, dim3(grid), dim3(256), args, 0, s));
 }
 
-int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, int dtrans, int R,
-                 double *BT, int NT, int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s)
+int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
+                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s)
 {
     StripArgs a;
-    a.A = A; a.D = D; a.dts = dts; a.doff = doff; a.ldd = ldd; a.dtrans = dtrans;
+    a.A = A; a.D = D; a.dts = dts; a.doff = doff; a.ldd = ldd;
     a.BT = BT; a.flags = flags;
     a.NT = NT; a.ldt = ldt; a.nrp = nrp;
-    a.S = NT * ldt / R;
+    a.S = NT * ldt / SR;
     a.trans = trans;
     a.info = info;
     int used = 0;
@@ -392,20 +406,11 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
         const int nc = (nrp - c0 >= 64) ? 4 : (nrp - c0) / 16;
         a.c0 = c0;
         a.epoch = epoch + used++;
-        if (R == 128) {
-            switch (nc) {
-            case 1: strip_launch<128, 1>(a, s); break;
-            case 2: strip_launch<128, 2>(a, s); break;
-            case 3: strip_launch<128, 3>(a, s); break;
-            default: strip_launch<128, 4>(a, s); break;
-            }
-        } else {
-            switch (nc) {
-            case 1: strip_launch<64, 1>(a, s); break;
-            case 2: strip_launch<64, 2>(a, s); break;
-            case 3: strip_launch<64, 3>(a, s); break;
-            default: strip_launch<64, 4>(a, s); break;
-            }
+        switch (nc) {
+        case 1: strip_launch<1>(a, s); break;
+        case 2: strip_launch<2>(a, s); break;
+        case 3: strip_launch<3>(a, s); break;
+        default: strip_launch<4>(a, s); break;
         }
     }
     return used;

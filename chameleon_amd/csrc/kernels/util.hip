@@ -246,38 +246,80 @@ void set_identity(double *d, int ntiles, int ldt, hipStream_t s)
 __global__ __launch_bounds__(256) void norm_kernel(const double *d, Geo g, int uplo, int trans, double *colsum,
                                                     double *rowsum, double *w)
 {
-    const int t = blockIdx.y;
+    /* One workgroup per tile.  Sums are formed per tile row (RI) and per
+     * tile column (CJ) over the stored triangle, then added to the logical
+     * row/column sums with one atomic per index: element (r, c) adds to
+     * rowsum[r] and colsum[c], and, as an off-diagonal element of a
+     * symmetric matrix, also to rowsum[c] and colsum[r]. */
+    const int t = blockIdx.x;
     const int m = t % g.mt, n = t / g.mt;
     const int vm = g.vm(m), vn = g.vn(n);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool sym = uplo != 123;
+    /* logical origin of the tile and the roles of its rows/columns */
+    const int r0 = trans ? n * g.nb : m * g.mb, c0 = trans ? m * g.mb : n * g.nb;   /* of (i,j) -> (r,c) */
+    auto stored = [&](int i, int j) {
+        const int r = trans ? r0 + j : r0 + i, c = trans ? c0 + i : c0 + j;
+        return uplo == 123 || (uplo == 122 ? r >= c : r <= c);
+    };
+    auto diagonal = [&](int i, int j) { return (trans ? r0 + j == c0 + i : r0 + i == c0 + j); };
+    /* skip tiles outside the stored triangle */
+    if (uplo != 123) {
+        const int rlo = trans ? r0 : r0, rhi = trans ? r0 + vn - 1 : r0 + vm - 1;
+        const int clo = c0, chi = trans ? c0 + vm - 1 : c0 + vn - 1;
+        if (uplo == 122 && rhi < clo) return;
+        if (uplo == 121 && rlo > chi) return;
+    }
     const double *tile = d + g.tile(m, n);
     double mx = 0.0, ss = 0.0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < vm * vn; e += gridDim.x * blockDim.x) {
-        const int i = e % vm, j = e / vm;
-        int r = m * g.mb + i, c = n * g.nb + j;
-        if (trans) { int tmp = r; r = c; c = tmp; }
-        const double v = fabs(tile[i + (size_t)j * g.ldt]);
-        if (uplo == 122 && r < c) continue;   /* Lower stored triangle */
-        if (uplo == 121 && r > c) continue;   /* Upper stored triangle */
-        if (uplo == 123 || r == c) {
-            atomicAdd(&colsum[c], v);
-            atomicAdd(&rowsum[r], v);
-            ss += v * v;
-        } else {
-            /* off-diagonal element of a symmetric matrix counts twice */
-            atomicAdd(&colsum[c], v);
-            atomicAdd(&colsum[r], v);
-            atomicAdd(&rowsum[r], v);
-            atomicAdd(&rowsum[c], v);
-            ss += 2.0 * v * v;
+    /* tile rows: coalesced across threads, one column at a time */
+    for (int i = tid; i < vm; i += 256) {
+        double s = 0.0, sd = 0.0;
+        for (int j = 0; j < vn; ++j) {
+            if (!stored(i, j)) continue;
+            const double v = fabs(tile[i + (size_t)j * g.ldt]);
+            s += v;
+            if (diagonal(i, j)) { sd += v; ss += v * v; } else ss += (sym ? 2.0 : 1.0) * v * v;
+            mx = fmax(mx, v);
         }
-        mx = fmax(mx, v);
+        /* tile row i is logical row r0 + i (no trans) or column c0 + i (trans) */
+        if (!trans) {
+            atomicAdd(&rowsum[r0 + i], s);
+            if (sym) atomicAdd(&colsum[r0 + i], s - sd);
+        } else {
+            atomicAdd(&colsum[c0 + i], s);
+            if (sym) atomicAdd(&rowsum[c0 + i], s - sd);
+        }
+    }
+    /* tile columns: one wavefront per column, lanes over rows */
+    for (int j = wave; j < vn; j += 4) {
+        double s = 0.0, sd = 0.0;
+        for (int i = lane; i < vm; i += 64) {
+            if (!stored(i, j)) continue;
+            const double v = fabs(tile[i + (size_t)j * g.ldt]);
+            s += v;
+            if (diagonal(i, j)) sd += v;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_down(s, off);
+            sd += __shfl_down(sd, off);
+        }
+        if (lane == 0) {
+            if (!trans) {
+                atomicAdd(&colsum[c0 + j], s);
+                if (sym) atomicAdd(&rowsum[c0 + j], s - sd);
+            } else {
+                atomicAdd(&rowsum[r0 + j], s);
+                if (sym) atomicAdd(&colsum[r0 + j], s - sd);
+            }
+        }
     }
     /* wave reduction then one atomic per wave */
     for (int off = 32; off > 0; off >>= 1) {
         mx = fmax(mx, __shfl_down(mx, off));
         ss += __shfl_down(ss, off);
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         atomicAdd(&w[1], ss);
         /* max of non-negative doubles via integer max on the bit pattern */
         atomicMax(reinterpret_cast<unsigned long long *>(&w[0]), (unsigned long long)__double_as_longlong(mx));
@@ -289,8 +331,7 @@ void norm_partials(const double *d, int mt, int nt, int ldt, int wdt, int mb, in
 {
     if (mt * nt == 0) return;
     Geo g = geo(mt, nt, ldt, wdt, mb, nb, M, N);
-    dim3 grid((ldt * wdt + 255) / 256 < 64 ? (ldt * wdt + 255) / 256 : 64, mt * nt);
-    hipLaunchKernelGGL(norm_kernel, grid, dim3(256), 0, s, d, g, uplo, trans, colsum, rowsum, w);
+    hipLaunchKernelGGL(norm_kernel, dim3(mt * nt), dim3(256), 0, s, d, g, uplo, trans, colsum, rowsum, w);
 }
 
 /* ------------------------------------------------------------------ */

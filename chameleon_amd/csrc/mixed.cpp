@@ -109,11 +109,12 @@ int backward_error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t
     const int NT = MA->mt, ldt = MA->ldt, nrp = MX->nt * MX->wdt, rows = NT * ldt;
     const double anrm = norm_inf_sym(A);
     const size_t vb = (size_t)nrp * rows;
-    double *w = (double *)workspace2((3 * vb + nrp) * sizeof(double));
-    double *XT = w, *BT = w + vb, *RT = w + 2 * vb, *nw = w + 3 * vb;
+    const size_t pv = k::residual_scratch(NT, ldt, nrp);
+    double *w = (double *)workspace2((3 * vb + nrp + pv) * sizeof(double));
+    double *XT = w, *BT = w + vb, *RT = w + 2 * vb, *nw = w + 3 * vb, *part = nw + nrp;
     k::rhs_transpose((double *)MX->d, XT, NT, ldt, MX->wdt, MX->nt, 1, s);
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
-    k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, s);
+    k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, part, s);
     std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
     std::vector<double> rn = colmax_host(RT, nrp, nrp, rows, nw);
     const double eps = std::ldexp(1.0, -53);
@@ -151,14 +152,15 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
     if (lp_ok) {
         const double anrm = norm_inf_sym(A);
         const double cte = anrm * eps * std::sqrt((double)N);
-        /* workspace2: working factor | panels (2) | BT | XT | RT | norms */
+        /* workspace2: working factor | panels (2) | BT | XT | RT | norms | residual pieces */
         const size_t abytes = MA->bytes;
         const size_t pbytes = (size_t)NT * ts * 4;
         const size_t vb = (size_t)nrp * rows;
-        char *w = (char *)workspace2(abytes + 2 * pbytes + (3 * vb + nrp) * sizeof(double));
+        const size_t pv = k::residual_scratch(NT, ldt, nrp);
+        char *w = (char *)workspace2(abytes + 2 * pbytes + (3 * vb + nrp + pv) * sizeof(double));
         double *Aw = (double *)w;
         void *P0 = w + abytes, *P1 = w + abytes + pbytes;
-        double *BT = (double *)(w + abytes + 2 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb;
+        double *BT = (double *)(w + abytes + 2 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb, *part = nw + nrp;
         CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
 
         PotrfPlan p;
@@ -182,7 +184,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
             CHX_HIP(hipMemcpyAsync(XT, BT, vb * sizeof(double), hipMemcpyDeviceToDevice, s));
             solve_bt(sp, Aw, NT, ldt, XT, nrp, st->d_info);
             for (int it = 0; it <= ITERMAX; ++it) {
-                k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, s);
+                k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, part, s);
                 std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
                 std::vector<double> rn = colmax_host(RT, nrp, nrp, rows, nw);
                 bool ok = true;

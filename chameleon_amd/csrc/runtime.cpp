@@ -325,7 +325,6 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     c->ib = env_int("CHAMELEON_INNER_BLOCK_SIZE", 48);
     c->lookahead = env_int("CHAMELEON_LOOKAHEAD", 1);
     c->translation = env_int("CHAMELEON_TRANSLATION_MODE", ChamInPlace);
-    c->solve_strip_rows = env_int("CHAMAMD_SOLVE_STRIP", 64) == 128 ? 128 : 64;
     c->sched = env_int("CHAMAMD_SCHED", 0);
     CHX_HIP(hipSetDevice(c->device));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
