@@ -175,9 +175,13 @@ void solve_coop(const double *A, const double *TN, const double *TT, double *BT,
  * after the block is done.  D_s goes to LDS before any wait.
  *
  * Publication is a per-strip flag set to this launch's epoch after a
- * device-scope release; readers poll the flags and acquire.  A flag of
+ * device-scope release (the strip's stores written back from L2).  Readers
+ * poll the flags with device-scope loads and read published blocks with
+ * device-scope loads as well, which see memory past the non-coherent L2
+ * lines of other XCDs; a device-scope acquire fence would invalidate the
+ * reader's whole L2 instead, ~6 us per chain link (measured).  A flag of
  * strip j is only set after strip j consumed every strip before it, so
- * seeing flag j also makes all earlier strips visible.  Strips are dealt
+ * seeing flag j also covers all earlier strips.  Strips are dealt
  * to workgroups in chain order (t = blockIdx.x, + gridDim.x, ...) and the
  * grid is launched cooperatively, so every strip a workgroup waits for is
  * owned by a resident workgroup that is either on it or on an earlier
@@ -276,14 +280,21 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            /* no device-scope acquire here: it would invalidate the whole
+             * L2 of this XCD (about 6 us per chain link, measured); the
+             * published blocks are read with device-scope (coherent) loads
+             * instead, like the flags, after the flag was seen */
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         };
         auto stage_load = [&](int j, v2d (&xv)[XV]) {
             const double *Xj = xblock(j);
 #pragma unroll
             for (int h = 0; h < XV; ++h) {
                 const int e = 2 * (tid + 256 * h), r = e / NCOL, c = e % NCOL;
-                xv[h] = *(const v2d *)&Xj[c + (size_t)r * nrp];
+                const unsigned long long *src = (const unsigned long long *)&Xj[c + (size_t)r * nrp];
+                const unsigned long long u0 = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long u1 = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                xv[h] = v2d{__longlong_as_double((long long)u0), __longlong_as_double((long long)u1)};
             }
         };
         auto stage_store = [&](const v2d (&xv)[XV]) {
