@@ -362,6 +362,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--kappas", default="1e2,1e4,1e6", help="condition numbers of the mixed-precision leg")
+    ap.add_argument("--mixed-nb", type=int, default=0, help="tile size of the mixed-precision leg (0: the FP64 one)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve", type=float, default=3.0, metavar="SECONDS",
                     help="arrival window of the serving leg (0: skip)")
@@ -398,7 +399,8 @@ def main():
                 "launches": up.get("launches"),
                 "avg_launch_ms": round(up["ms"] / up["launches"], 4) if up.get("launches") else None}
     slat = None if args.no_latency else solve_latency(ch, n, nb, 30)
-    mixed = None if args.no_mixed else mixed_leg(ch, n, nrhs, nb, tuple(float(x) for x in args.kappas.split(",")))
+    mixed = None if args.no_mixed else mixed_leg(ch, n, nrhs, args.mixed_nb or nb,
+                                                 tuple(float(x) for x in args.kappas.split(",")))
     batch = None if args.no_batch else batch_leg(ch)
     serve = serve_leg(ch, args.serve) if args.serve > 0 else None
     cpu = None
