@@ -47,6 +47,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per trailing-update launch measured with the FETCH_SIZE / WRITE_SIZE
+# counters (scaled by a plain copy's readings) for the workload they were taken on
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "trailing_update_pmc_dposv_n32768.txt")
+TRAFFIC_WORKLOAD = (32768, 512)
+
+
+def measured_traffic(n, nb):
+    if (n, nb) != TRAFFIC_WORKLOAD or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 METRIC = "dposv TFLOP/s @n=32k + p99 solve latency, FP64 vs mixed-prec+IR, equal bwd error"
 
 
@@ -395,7 +408,8 @@ def main():
     achieved = up["flops"] / (up["ms"] * 1e-3) / 1e12 if up.get("ms") else None
     roofline = {"bound": "mfma", "kernel": "trailing_update_kernel (DSYRK+DGEMM)",
                 "achieved": round(achieved, 3) if achieved else None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4) if achieved else None, "traffic": None,
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4) if achieved else None,
+                "traffic": measured_traffic(n, nb), "traffic_unit": "bytes/launch",
                 "launches": up.get("launches"),
                 "avg_launch_ms": round(up["ms"] / up["launches"], 4) if up.get("launches") else None}
     slat = None if args.no_latency else solve_latency(ch, n, nb, 30)
