@@ -103,7 +103,7 @@ def gpu_dposv(ch, n, nrhs, nb, steps, warmup, stats=True):
                   file=sys.stderr, flush=True)
     kstats = {}
     if stats:
-        names = {0: "update", 1: "potrf", 2: "trsm", 3: "solve"}
+        names = {0: "update", 1: "potrf", 2: "trsm", 3: "solve", 4: "panel"}
         for cls, name in names.items():
             la, ms, fl = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
             L.CHAMAMD_Kernel_Stats(cls, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl))
@@ -202,7 +202,7 @@ def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
                 tm = time.perf_counter() - t0
                 seq.destroy()
                 kst = {}
-                for cls, name in {0: "update", 1: "potrf", 2: "trsm", 3: "solve"}.items():
+                for cls, name in {0: "update", 1: "potrf", 2: "trsm", 3: "solve", 4: "panel"}.items():
                     la, ms, fl = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
                     L.CHAMAMD_Kernel_Stats(cls, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl))
                     kst[name] = {"launches": la.value, "ms": round(ms.value, 2),

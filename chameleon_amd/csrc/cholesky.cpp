@@ -89,10 +89,26 @@ void potrf_run(const PotrfPlan &p)
         const long long S = NT - k - 1;                           /* columns of bulk update k-1 */
         return (S * (S + 1) / 2) * bpt >= 2LL * ncu ? 4 : 8;
     };
+    const bool fused = c->panel && ldt % 64 == 0;
     auto panel = [&](int k) {
         const int nvalid = (k == NT - 1) ? p.nvalid_last : p.nb;
         double *dk = dinv + (size_t)k * dper;
         const double nk = tsz(k);
+        if (fused) {
+            /* POTRF(k) and the TRSM of the tiles below in one launch */
+            double f = nk * (nk * nk / 3.0 + nk / 2.0 + 1.0 / 6.0);
+            for (int m = k + 1; m < NT; ++m) f += tsz(m) * nbd * nbd;
+            const int ND = ldt / 64;
+            int epoch = 0;
+            int *flags = strip_flags(ND * ND, 1, &epoch);
+            unsigned *ctr = panel_counters();
+            timed(CHAMAMD_KERNEL_PANEL, f, sm, [&] {
+                k::panel_factor(tile(k, k), tile(std::min(k + 1, NT - 1), k), (long long)ts, NT - k - 1, ldt, dk,
+                                flags, ctr, epoch, p.d_info, p.nb * k, nvalid, sm);
+            });
+            if (k + 1 < NT && p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, p.P[k & 1], p.lp_mode, sm);
+            return;
+        }
         const int waves = potrf_waves(k);
         timed(CHAMAMD_KERNEL_POTRF, nk * (nk * nk / 3.0 + nk / 2.0 + 1.0 / 6.0), sm,
               [&] { k::potrf_diag(tile(k, k), ldt, dk, p.nb * k, nvalid, p.d_info, waves, sm); });

@@ -24,10 +24,14 @@
 #include <cstdint>
 #include <vector>
 #include "chameleon.h"
+#include "chameleon_amd.h"
 
 namespace chx {
 
 void fatal_hip(hipError_t e, const char *what, const char *file, int line);
+/* info value left by a kernel whose bounded wait for another workgroup ran
+ * out (an internal error, never a property of the matrix) */
+#define PANEL_STALLED (-(1 << 30))
 #define CHX_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) ::chx::fatal_hip(_e, #call, __FILE__, __LINE__); } while (0)
 
 void error(const char *func, const char *msg);
@@ -76,10 +80,12 @@ struct Context {
     int *strip_flags = nullptr;
     int strip_flags_n = 0;
     int strip_epoch = 0;
+    unsigned *panel_ctr = nullptr;  /* ticket / finished counters of the fused panel kernel */
+    int panel = 1;                  /* fused panel kernel (CHAMAMD_PANEL=0: POTRF + TRSM)   */
     int sched = 0;                /* lookahead issue order (CHAMAMD_SCHED) */
-    long long st_launches[4] = {0, 0, 0, 0};
-    double st_ms[4] = {0, 0, 0, 0};
-    double st_flops[4] = {0, 0, 0, 0};
+    long long st_launches[CHAMAMD_KERNEL_NCLASS] = {};
+    double st_ms[CHAMAMD_KERNEL_NCLASS] = {};
+    double st_flops[CHAMAMD_KERNEL_NCLASS] = {};
 };
 
 Context *ctx();                       /* nullptr when not initialised */
@@ -104,6 +110,7 @@ inline void timed(int cls, double flops, hipStream_t s, F body)
 void *workspace(size_t bytes);        /* context workspace of >= bytes */
 void *workspace2(size_t bytes);       /* second, independent workspace     */
 int *strip_flags(int n, int launches, int *epoch);
+unsigned *panel_counters();
 
 Mirror *mirror_of(const CHAM_desc_t *A);
 /* Make the device mirror of A hold its current values in orientation
@@ -177,6 +184,8 @@ void lacpy_tiles(const double *src, double *dst, int mt, int nt, int ldt, int wd
                  int uplo, hipStream_t s);
 /* chol.hip */
 void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *info, int waves, hipStream_t s);
+void panel_factor(double *T, double *P, long long ts, int S, int ldt, double *dinv, int *flags, unsigned *ctr,
+                  int epoch, int *info, int iinfo, int nvalid, hipStream_t s);
 void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int ldt, const double *Lkk,
                             const double *dinv, const int *info, hipStream_t s);
 void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, int ldt, const double *L0,

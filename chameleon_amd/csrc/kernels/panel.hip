@@ -577,5 +577,361 @@ void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int l
     trsm_right_lower_trans_batched(panel, ntiles, (long long)tile_stride, ldt, Lkk, 0, dinv, 0, info, s);
 }
 
+/* ================================================================== */
+/* Fused panel factorisation of step k: POTRF of the diagonal tile
+ * (pzpotrf.c:90-94) and the TRSM of the S tiles below it (pzpotrf.c:99-105)
+ * in one launch, spread over 64-row units so that the panel is not bounded
+ * by one compute unit.
+ *
+ * Units (one workgroup of 4 wavefronts each, numbered in the order the
+ * workgroups start, through a ticket counter):
+ *   r < ND      owner of row block r of the diagonal tile (ND = ldt / 64)
+ *   r >= ND     one 64-row block of a tile below
+ * Every unit walks the 64-column blocks C of the panel left-looking:
+ *   R = A(u, C) - sum_{p<C} X(u, p) X(C, p)^T      X(C, .) published by owner C
+ *   X(u, C) = R Linv_C^T                           Linv_C published by owner C
+ * and owner r, after its blocks C < r, factors its diagonal block
+ *   L(r, r) L(r, r)^T = A(r, r) - sum_{p<r} X(r, p) X(r, p)^T
+ * and publishes L(r, r)'s inverse.  The SYRK part over p < r-1 is formed
+ * before the wait for Linv_{r-1}, so the chain per column block is one
+ * 64 x 64 product, one rank-64 update and the 64 x 64 factorisation.
+ *
+ * Publication: flag F[r * ND + C] set to this launch's epoch once owner r
+ * stored X(r, C) (C < r) or L(r, r) and Linv_r (C = r), after a
+ * device-scope release.  Published blocks are read with device-scope
+ * loads (past the non-coherent L2 of other XCDs).  A unit only waits for
+ * flags of owners, and owners only for owners with smaller tickets, which
+ * have started: the waits end.  A wait also ends when the info flag is
+ * set (a failed pivot) or after a bounded number of polls.             */
+
+#define PF_SPIN_LIMIT (1 << 22)
+
+struct PanelArgs {
+    double *T;           /* diagonal tile                                         */
+    double *P;           /* first tile below it; tile t at P + t * ts             */
+    long long ts;
+    int S, ldt, ND;
+    double *dinv;        /* Linv_C at dinv + C * 4096, column-major 64 x 64        */
+    int *flags;          /* ND x ND publication flags                              */
+    unsigned *ctr;       /* [0] tickets taken, [1] workgroups finished (both zero
+                          * between launches: the last workgroup clears them)      */
+    int epoch;
+    int *info;
+    int iinfo, nvalid;
+};
+
+/* Buffer accesses relative to a tile: one per-lane offset register, the
+ * uniform part of each address in a scalar offset.  aux 16 sets sc1, a
+ * device-scope (coherent) access, the form of a relaxed device-scope
+ * atomic load. */
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t pf_rsrc(const double *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ double pf_ld(rsrc_t r, int vo, int so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ double pf_ld_dev(rsrc_t r, int vo, int so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 16));
+}
+__device__ __forceinline__ void pf_st(double v, rsrc_t r, int vo, int so)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, vo, so, 0);
+}
+
+/* Wait for flag idx of this launch; false if the factorisation failed
+ * meanwhile (or the poll limit was reached).  Uniform over the workgroup. */
+__device__ bool pf_wait(const PanelArgs &a, int idx, int tid, int *s_bad)
+{
+    if (tid == 0) {
+        int bad = 0, n = 0;
+        while (__hip_atomic_load(&a.flags[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.epoch) {
+            if (__hip_atomic_load(a.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { bad = 1; break; }
+            if (++n > PF_SPIN_LIMIT) {
+                atomicCAS(a.info, 0, PANEL_STALLED);
+                bad = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        *s_bad = bad;
+    }
+    __syncthreads();
+    const int bad = *s_bad;
+    __syncthreads();
+    return bad == 0;
+}
+
+__device__ __forceinline__ void pf_publish(const PanelArgs &a, int idx, int tid)
+{
+    __syncthreads();
+    if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&a.flags[idx], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+/* acc[f] += X[16w + i, 0:64n] Y[16f + j, 0:64n]^T (wave w, lane (i, j) of
+ * the result layout).  X: the unit's own rows (plain loads); Y: 64 rows
+ * read with device-scope loads and staged in LDS once per 64-column block;
+ * the next block is loaded while the current one is multiplied.  */
+__device__ void pf_kloop(const double *X, const double *Y, int ldt, int n, double *Ys, int tid, int wave, int lane,
+                         v4d (&acc)[4])
+{
+    const int li = lane & 15, lk = lane >> 4;
+    const rsrc_t rx = pf_rsrc(X), ry = pf_rsrc(Y);
+    const int vx = (16 * wave + li + lk * ldt) * 8;            /* X(16w + li, lk)          */
+    const int vy = ((tid & 63) + (tid >> 6) * ldt) * 8;        /* Y(tid % 64, tid / 64)    */
+    const int c4 = 4 * ldt * 8;                                 /* four columns, in bytes   */
+    double x[16], yst[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = pf_ld(rx, vx, q * c4);
+#pragma unroll
+    for (int h = 0; h < 16; ++h) yst[h] = pf_ld_dev(ry, vy, h * c4);
+    for (int p = 0; p < n; ++p) {
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 16; ++h) Ys[tid + 256 * h] = yst[h];
+        __syncthreads();
+        const bool more = p + 1 < n;
+        const int o = (p + 1) * 16 * c4;
+        if (more) {
+#pragma unroll
+            for (int h = 0; h < 16; ++h) yst[h] = pf_ld_dev(ry, vy, o + h * c4);
+        }
+        /* x[q] is consumed by its four products, then refilled with the
+         * next block's value */
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double xq = x[q];
+            if (more) x[q] = pf_ld(rx, vx, o + q * c4);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) acc[f] = mfma_f64(Ys[(4 * q + lk) * 64 + 16 * f + li], xq, acc[f]);
+        }
+    }
+}
+
+/* Factor the 64 x 64 block in Ld (lower part, zeros above) in place and
+ * build its inverse in Lv: the 16-column steps of potrf_jobs_kernel's (b)
+ * and (c) phases for 4 wavefronts.  Returns the 1-based column of the
+ * first non-positive pivot, 0 if none (uniform). */
+__device__ int pf_factor64(double (*Ld)[OBP], double (*Lb)[16][17], double (*Lv)[OBP], double *Rd, int *s_fail,
+                           int wave, int lane)
+{
+    const int li = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x;
+    if (tid == 0) *s_fail = 0;
+    __syncthreads();
+    for (int cb = 0; cb < 4; ++cb) {
+        const int c0 = 16 * cb;
+        if (cb > 0) {
+            const int rb = wave;
+            if (rb < 4 - cb) {
+                const int r0 = c0 + 16 * rb;
+                v4d acc = zero4d();
+                for (int p = 0; p < c0; p += 4) acc = mfma_f64(Ld[c0 + li][p + lk], Ld[r0 + li][p + lk], acc);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int c = lk + 4 * s;
+                    if (rb > 0 || li >= c) Ld[r0 + li][c0 + c] -= acc[s];
+                }
+            }
+            __syncthreads();
+        }
+        if (wave == 0) {
+            double a[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) a[q] = (q <= li) ? Ld[c0 + li][c0 + q] : 0.0;
+            int fail;
+            double rd;
+            block16_potrf(a, lane, fail, rd);
+            if (lane < 16) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) Ld[c0 + li][c0 + q] = (q <= li) ? a[q] : 0.0;
+                Rd[li] = rd;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            double col[16];
+            block16_inverse_col(&Ld[c0][c0], OBP, Rd, lane, col);
+            if (lane < 16) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) Lb[cb][q][li] = col[q];
+            }
+            if (lane == 0 && fail && *s_fail == 0) *s_fail = c0 + fail;
+        }
+        __syncthreads();
+        if (*s_fail) return *s_fail;
+        {
+            const int rb = wave;
+            if (rb < 4 - cb - 1) {
+                const int r0 = c0 + 16 + 16 * rb;
+                double pv[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) pv[kk] = Ld[r0 + li][c0 + 4 * kk + lk];
+                v4d acc = zero4d();
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) acc = mfma_f64(Lb[cb][li][4 * kk + lk], pv[kk], acc);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) Ld[r0 + li][c0 + lk + 4 * s] = acc[s];
+            }
+        }
+        __syncthreads();
+    }
+    assemble_inverse(Ld, Lb, Lv, 4, wave, lane);
+    return 0;
+}
+
+__device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys, double (*Ld)[OBP],
+                                           double (*Lb)[16][17], double *Rd, int *s_int)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ND = a.ND, ldt = a.ldt;
+    const bool owner = t < ND;
+    const int r = owner ? t : (t - ND) % ND;
+    double *R = owner ? a.T + 64 * r : a.P + (size_t)((t - ND) / ND) * a.ts + 64 * r;
+    const int nsteps = owner ? r : ND;
+    const int rowo = 16 * wave + li;
+    const rsrc_t rR = pf_rsrc(R);
+    const int vr = (rowo + lk * ldt) * 8;          /* R(rowo, lk) */
+    /* owner r keeps sum_p X(r, p) X(r, p)^T of its diagonal block in Ld
+     * ([row][col]) and the last X block in Ys ([k][row]) */
+    for (int C = 0; C < nsteps; ++C) {
+        v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+        if (C > 0) {
+            if (!pf_wait(a, C * ND + C - 1, tid, s_int)) return;
+            pf_kloop(R, a.T + 64 * C, ldt, C, Ys, tid, wave, lane, acc);
+        }
+        if (owner && C == r - 1) {
+            /* SYRK part of the diagonal block over p < r-1, before the wait */
+            v4d accd[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+            if (C > 0) pf_kloop(R, R, ldt, C, Ys, tid, wave, lane, accd);
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) Ld[rowo][16 * f + lk + 4 * s] = accd[f][s];
+        }
+        double av[4][4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) av[f][s] = pf_ld(rR, vr, (64 * C + 16 * f + 4 * s) * ldt * 8) - acc[f][s];
+        if (!pf_wait(a, C * ND + C, tid, s_int)) return;
+        const rsrc_t rL = pf_rsrc(a.dinv + (size_t)C * 4096);
+        double lv[16];
+#pragma unroll
+        for (int h = 0; h < 16; ++h) lv[h] = pf_ld_dev(rL, tid * 8, h * 2048);
+#pragma unroll
+        for (int h = 0; h < 16; ++h) Ys[tid + 256 * h] = lv[h];
+        __syncthreads();
+        v4d xo[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            xo[f] = zero4d();
+#pragma unroll
+            for (int kq = 0; kq < 16; ++kq)
+                if (kq < 4 * (f + 1)) xo[f] = mfma_f64(Ys[(4 * kq + lk) * 64 + 16 * f + li], av[kq >> 2][kq & 3], xo[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) pf_st(xo[f][s], rR, vr, (64 * C + 16 * f + 4 * s) * ldt * 8);
+        if (owner) {
+            pf_publish(a, r * ND + C, tid);
+            if (C == r - 1) {
+                /* the last X block, [k][row], for the rank-64 term */
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) Ys[(16 * f + lk + 4 * s) * 64 + rowo] = xo[f][s];
+            }
+        }
+    }
+    if (!owner) return;
+
+    /* diagonal block of owner r: A(r, r) - sum_p X(r, p) X(r, p)^T */
+    double (*Lv)[OBP] = reinterpret_cast<double (*)[OBP]>(Ys);
+    v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+    __syncthreads();
+    if (r > 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double xq = Ys[(4 * q + lk) * 64 + rowo];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) acc[f] = mfma_f64(Ys[(4 * q + lk) * 64 + 16 * f + li], xq, acc[f]);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int c = 16 * f + lk + 4 * s;
+            const double v = pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8) - acc[f][s] - (r > 0 ? Ld[rowo][c] : 0.0);
+            Ld[rowo][c] = (rowo >= c) ? v : 0.0;
+        }
+    __syncthreads();
+    const int fail = pf_factor64(Ld, Lb, Lv, Rd, s_int, wave, lane);
+    if (fail) {
+        const int col = 64 * r + fail;
+        if (col - 1 < a.nvalid) {
+            if (tid == 0 && __hip_atomic_load(a.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                __hip_atomic_store(a.info, a.iinfo + col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
+    double *Tr = a.T + 64 * r + (size_t)(64 * r) * ldt;
+    double *Dv = a.dinv + (size_t)r * 4096;
+    for (int e = tid; e < OB * OB; e += 256) {
+        const int rr = e % OB, cc = e / OB;
+        if (rr >= cc) Tr[rr + (size_t)cc * ldt] = Ld[rr][cc];
+        Dv[e] = Lv[rr][cc];
+    }
+    pf_publish(a, r * ND + r, tid);
+}
+
+__global__ __launch_bounds__(256, 2) void panel_factor_kernel(PanelArgs a)
+{
+    __shared__ double Ys[OB * OBP];       /* Y blocks, Linv_C, the inverse being built */
+    __shared__ double Ld[OB][OBP];        /* diagonal block, the last X block          */
+    __shared__ double Lb[4][16][17];
+    __shared__ double Rd[16];
+    __shared__ int s_int, s_ticket, s_info;
+    const int tid = threadIdx.x;
+    __builtin_amdgcn_s_setprio(2);        /* critical path: win issue next to update waves */
+    if (tid == 0) {
+        s_ticket = (int)atomicAdd(&a.ctr[0], 1u);
+        s_info = __hip_atomic_load(a.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    /* a ticket past the grid would mean the counters were not cleared */
+    if (s_ticket >= a.ND * (a.S + 1)) {
+        if (tid == 0) atomicCAS(a.info, 0, PANEL_STALLED);
+    } else if (s_info == 0) {
+        panel_unit(a, s_ticket, Ys, Ld, Lb, Rd, &s_int);
+    }
+    __syncthreads();
+    if (tid == 0 && atomicAdd(&a.ctr[1], 1u) == gridDim.x - 1) {
+        __hip_atomic_store(&a.ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.ctr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+void panel_factor(double *T, double *P, long long ts, int S, int ldt, double *dinv, int *flags, unsigned *ctr,
+                  int epoch, int *info, int iinfo, int nvalid, hipStream_t s)
+{
+    PanelArgs a;
+    a.T = T; a.P = P; a.ts = ts; a.S = S; a.ldt = ldt; a.ND = ldt / 64;
+    a.dinv = dinv; a.flags = flags; a.ctr = ctr; a.epoch = epoch;
+    a.info = info; a.iinfo = iinfo; a.nvalid = nvalid;
+    const int grid = a.ND * (S + 1);
+    hipLaunchKernelGGL(panel_factor_kernel, dim3(grid), dim3(256), 0, s, a);
+}
+
 } /* namespace k */
 } /* namespace chx */

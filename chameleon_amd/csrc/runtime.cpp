@@ -73,6 +73,16 @@ int *strip_flags(int n, int launches, int *epoch)
     return c->strip_flags;
 }
 
+unsigned *panel_counters()
+{
+    Context *c = g_ctx;
+    if (!c->panel_ctr) {
+        CHX_HIP(hipMalloc(&c->panel_ctr, 2 * sizeof(unsigned)));
+        CHX_HIP(hipMemsetAsync(c->panel_ctr, 0, 2 * sizeof(unsigned), c->s_main));
+    }
+    return c->panel_ctr;
+}
+
 void *workspace2(size_t bytes)
 {
     Context *c = g_ctx;
@@ -276,7 +286,12 @@ int sequence_wait_internal(RUNTIME_sequence_t *seq)
     SeqState *st = seq_state(seq);
     if (st && st->pending_info) {
         CHX_HIP(hipMemcpy(st->h_info, st->d_info, sizeof(int), hipMemcpyDeviceToHost));
-        if (*st->h_info > 0 && seq->status == CHAMELEON_SUCCESS) seq->status = *st->h_info;
+        if (*st->h_info == PANEL_STALLED) {
+            warning("CHAMELEON_Sequence_Wait", "internal error: a panel kernel wait ran out");
+            if (seq->status == CHAMELEON_SUCCESS) seq->status = CHAMELEON_ERR_UNEXPECTED;
+        } else if (*st->h_info > 0 && seq->status == CHAMELEON_SUCCESS) {
+            seq->status = *st->h_info;
+        }
         st->pending_info = false;
     }
     return seq->status;
@@ -326,6 +341,7 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     c->lookahead = env_int("CHAMELEON_LOOKAHEAD", 1);
     c->translation = env_int("CHAMELEON_TRANSLATION_MODE", ChamInPlace);
     c->sched = env_int("CHAMAMD_SCHED", 0);
+    c->panel = env_int("CHAMAMD_PANEL", 1);
     CHX_HIP(hipSetDevice(c->device));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_update, hipStreamNonBlocking));
@@ -351,6 +367,7 @@ int __chameleon_finalize(void)
     if (c->ws) CHX_HIP(hipFree(c->ws));
     if (c->ws2) CHX_HIP(hipFree(c->ws2));
     if (c->strip_flags) CHX_HIP(hipFree(c->strip_flags));
+    if (c->panel_ctr) CHX_HIP(hipFree(c->panel_ctr));
     if (c->pinned) CHX_HIP(hipHostFree(c->pinned));
     CHX_HIP(hipFree(c->d_info));
     CHX_HIP(hipStreamDestroy(c->s_main));

@@ -21,7 +21,8 @@ enum {
     CHAMAMD_KERNEL_POTRF  = 1,   /* diagonal tile POTRF                             */
     CHAMAMD_KERNEL_TRSM   = 2,   /* panel TRSM                                      */
     CHAMAMD_KERNEL_SOLVE  = 3,   /* triangular-solve GEMMs (forward + backward)     */
-    CHAMAMD_KERNEL_NCLASS = 4
+    CHAMAMD_KERNEL_PANEL  = 4,   /* fused diagonal POTRF + panel TRSM of one step   */
+    CHAMAMD_KERNEL_NCLASS = 5
 };
 
 /* Accumulated launches, summed device time (ms) and algorithmic flops

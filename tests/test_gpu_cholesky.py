@@ -209,6 +209,29 @@ def test_non_spd_info(ch, golden, key, n, bump, seed, nb, uplo):
     assert ch.lib.CHAMELEON_dposv(uplo, n, 2, _p(A), n, _p(B), n) == int(golden[key][0])
 
 
+@pytest.mark.parametrize("nb", [64, 128, 320])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_non_spd_info_fused_panel(ch, nb, uplo):
+    """A diagonally dominant SPD matrix with one diagonal entry set to -1 has
+    its first non-positive leading minor exactly there: info = j + 1.  The
+    positions fall in the first, a middle and the last tile, at different
+    64-column blocks of the multi-workgroup panel factorisation."""
+    n = 700
+    for j in (5, 200, 650):
+        say("non-spd fused", nb, uplo, j)
+        ch.set_param(ch.TILE_SIZE, nb)
+        A = oracle.dplgsy(float(n), UL_, n, 51)
+        A[j, j] = -1.0
+        assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n) == j + 1
+        A = oracle.dplgsy(float(n), UL_, n, 51)
+        A[j, j] = -1.0
+        B = np.ones((n, 3), order="F")
+        assert ch.lib.CHAMELEON_dposv(uplo, n, 3, _p(A), n, _p(B), n) == j + 1
+    # the context is usable afterwards
+    A = oracle.dplgsy(float(n), UL_, n, 51)
+    assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n) == 0
+
+
 # ---------------------------------------------------------- large tile sizes
 @pytest.mark.parametrize("n,nb,nrhs", [(1000, 128, 16), (1000, 320, 64), (2048, 256, 64), (2100, 384, 64),
                                        (2048, 512, 1), (1536, 192, 3)])
