@@ -32,6 +32,16 @@ __device__ __forceinline__ v4f mfma_f32(float a, float b, v4f c)
 __device__ __forceinline__ int mfma64_row(int l, int s) { return (l >> 4) + 4 * s; }
 __device__ __forceinline__ int mfma32_row(int l, int s) { return 4 * (l >> 4) + s; }
 
+/* Value of v in lane l (uniform l), as a scalar broadcast: the readlane
+ * builtin moves 32 bits, so the double travels as its two halves. */
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ v4d zero4d() { v4d z = {0.0, 0.0, 0.0, 0.0}; return z; }
 __device__ __forceinline__ v4f zero4f() { v4f z = {0.f, 0.f, 0.f, 0.f}; return z; }
 

@@ -91,9 +91,10 @@ __device__ __forceinline__ double rsqrt_f64(double d)
 
 /* Cholesky of a 16 x 16 block by one wavefront: lane i (of each 16-lane
  * group; the four groups compute the same thing) holds row i in a[];
- * pivots and column entries travel by lane shuffles.  On return a[] holds
- * row i of L and rd = 1 / L(i,i).  `fail` is the 1-based index of the
- * first non-positive pivot (uniform), 0 if none. */
+ * pivots and column entries are read from lanes 0-15 with readlane (a
+ * scalar broadcast, no LDS round trip).  On return a[] holds row i of L
+ * and rd = 1 / L(i,i).  `fail` is the 1-based index of the first
+ * non-positive pivot (uniform), 0 if none. */
 __device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fail, double &rd)
 {
     const int i = lane & 15;
@@ -101,7 +102,7 @@ __device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fa
     rd = 0.0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const double d = __shfl(a[j], j, 16);
+        const double d = readlane_f64(a[j], j);
         if (fail == 0 && !(d > 0.0)) fail = j + 1;
         const double r = rsqrt_f64(d);
         const double lij = a[j] * r;                      /* l_ij for i > j, sqrt(d) for i == j */
@@ -109,7 +110,7 @@ __device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fa
         if (i >= j) a[j] = lij;
 #pragma unroll
         for (int k = j + 1; k < 16; ++k) {
-            const double lkj = __shfl(lij, k, 16);
+            const double lkj = readlane_f64(lij, k);
             if (i > j && k <= i) a[k] -= lij * lkj;
         }
     }
@@ -800,8 +801,8 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
     const int rowo = 16 * wave + li;
     const rsrc_t rR = pf_rsrc(R);
     const int vr = (rowo + lk * ldt) * 8;          /* R(rowo, lk) */
-    /* owner r keeps sum_p X(r, p) X(r, p)^T of its diagonal block in Ld
-     * ([row][col]) and the last X block in Ys ([k][row]) */
+    /* owner r keeps A(r, r) - sum_{p<r-1} X(r, p) X(r, p)^T of its diagonal
+     * block in Ld ([row][col]) and the last X block in Ys ([k][row]) */
     for (int C = 0; C < nsteps; ++C) {
         v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
         if (C > 0) {
@@ -815,7 +816,8 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
 #pragma unroll
             for (int f = 0; f < 4; ++f)
 #pragma unroll
-                for (int s = 0; s < 4; ++s) Ld[rowo][16 * f + lk + 4 * s] = accd[f][s];
+                for (int s = 0; s < 4; ++s)
+                    Ld[rowo][16 * f + lk + 4 * s] = pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8) - accd[f][s];
         }
         double av[4][4];
 #pragma unroll
@@ -872,7 +874,7 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int c = 16 * f + lk + 4 * s;
-            const double v = pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8) - acc[f][s] - (r > 0 ? Ld[rowo][c] : 0.0);
+            const double v = (r > 0 ? Ld[rowo][c] : pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8)) - acc[f][s];
             Ld[rowo][c] = (rowo >= c) ? v : 0.0;
         }
     __syncthreads();
