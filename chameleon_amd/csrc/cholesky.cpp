@@ -214,7 +214,7 @@ static void solve_pair(const SolvePlan &p, double *A, int NT, int ldt, double *B
     const int S = NT * ldt / 64;
     const int chunks = (nrp + 63) / 64;
     int epoch = 0;
-    int *flags = strip_flags(S, 2 * chunks, &epoch);
+    int *flags = strip_flags(4 * S, 2 * chunks, &epoch);    /* up to 4 column chunks side by side */
     const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
     /* 64 x 64 inverse of block b of diagonal tile t at dinv + (t*nob + b)*4096, ld 64 */
     for (int dir = 0; dir < 2; ++dir)

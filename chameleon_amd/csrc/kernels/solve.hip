@@ -197,8 +197,9 @@ struct StripArgs {
     long long dts, doff;
     int ldd;
     double *BT;           /* right-hand sides, element (c, r) at c + r * nrp         */
-    int *flags;           /* one per strip                                            */
-    int NT, ldt, nrp, c0; /* c0: first column of this launch's chunk                  */
+    int *flags;           /* one per strip and column chunk: chunk c at flags + c*S   */
+    int NT, ldt, nrp, c0; /* c0: first column of this launch's chunks                 */
+    int nchunk;           /* column chunks of 16*NC columns solved side by side       */
     int S;                /* number of strips                                         */
     int trans, epoch;
     const int *info;
@@ -239,16 +240,22 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
     const int wr = wave * 16;
     const bool fwd = a.trans == 0;
     const bool lvec = !fwd;       /* backward reads L(j, s)^T as row vectors */
-    for (int t = blockIdx.x; t < a.S; t += gridDim.x) {
-        const int s = fwd ? t : a.S - 1 - t;
+    /* task t = (strip in chain order) * nchunk + chunk: the chunks are
+     * independent chains, interleaved so that the tasks a workgroup waits
+     * for always have smaller indices */
+    for (int t = blockIdx.x; t < a.S * a.nchunk; t += gridDim.x) {
+        const int si = t / a.nchunk, cc = t - si * a.nchunk;
+        const int s = fwd ? si : a.S - 1 - si;
+        const int c0 = a.c0 + cc * NCOL;
+        int *const flags = a.flags + (size_t)cc * a.S;
         const int tm = s / q, rs = (s % q) * SR;
-        double *Bs = a.BT + a.c0 + (size_t)s * SR * nrp;
+        double *Bs = a.BT + c0 + (size_t)s * SR * nrp;
         auto block_of = [&](int j) -> const double * {
             const int tj = j / q, rj = (j % q) * SR;
             return fwd ? a.A + ((size_t)tj * a.NT + tm) * ts + rs + (size_t)rj * ldt
                        : a.A + ((size_t)tm * a.NT + tj) * ts + rj + (size_t)rs * ldt;
         };
-        auto xblock = [&](int j) -> const double * { return a.BT + a.c0 + (size_t)j * SR * nrp; };
+        auto xblock = [&](int j) -> const double * { return a.BT + c0 + (size_t)j * SR * nrp; };
 
         v4d acc[NC];
 #pragma unroll
@@ -271,7 +278,7 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
                 const int jj = fwd ? j + lane : j - lane;
                 const bool inr = fwd ? jj < s : jj > s;
                 const bool set =
-                    inr && __hip_atomic_load(&a.flags[jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+                    inr && __hip_atomic_load(&flags[jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
                 const unsigned long long m = __ballot(set);
                 if (m & 1ull) {
                     const int run = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
@@ -379,7 +386,7 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
         __syncthreads();
         if (tid == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&a.flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -396,7 +403,7 @@ static void strip_launch(const StripArgs &a, hipStream_t s)
         if (per_cu < 1) per_cu = 1;
     }
     int grid = ncu * per_cu;
-    if (grid > a.S) grid = a.S;
+    if (grid > a.S * a.nchunk) grid = a.S * a.nchunk;
     StripArgs x = a;
     void *args[] = {&x};
     CHX_HIP(hipLaunchCooperativeKernel((const void *)strip_solve_kernel<NC>, dim3(grid), dim3(256), args, 0, s));
@@ -412,11 +419,26 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
     a.S = NT * ldt / SR;
     a.trans = trans;
     a.info = info;
+    /* groups of up to 64 columns per launch; a group is split into chunks
+     * of 16 * SOLVE_NC columns that run as separate chains side by side
+     * (shorter work per chain link; the factor rows are read once per
+     * chunk, mostly from the shared cache) */
+    static const int snc = [] {
+        const char *e = getenv("CHAMAMD_SOLVE_NC");
+        const int v = e ? atoi(e) : 2;
+        return (v == 1 || v == 2 || v == 4) ? v : 2;
+    }();
     int used = 0;
     for (int c0 = 0; c0 < nrp; c0 += 64) {
-        const int nc = (nrp - c0 >= 64) ? 4 : (nrp - c0) / 16;
+        const int cols = (nrp - c0 >= 64) ? 64 : nrp - c0;
         a.c0 = c0;
         a.epoch = epoch + used++;
+        int nc = cols / 16;
+        a.nchunk = 1;
+        if (cols == 64 && snc < 4) {
+            nc = snc;
+            a.nchunk = 4 / snc;
+        }
         switch (nc) {
         case 1: strip_launch<1>(a, s); break;
         case 2: strip_launch<2>(a, s); break;
