@@ -103,7 +103,7 @@ def gpu_dposv(ch, n, nrhs, nb, steps, warmup, stats=True):
                   file=sys.stderr, flush=True)
     kstats = {}
     if stats:
-        names = {0: "update", 1: "potrf", 2: "trsm", 3: "solve", 4: "panel"}
+        names = {0: "update", 1: "potrf", 2: "trsm", 3: "solve", 4: "panel", 5: "colupd"}
         for cls, name in names.items():
             la, ms, fl = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
             L.CHAMAMD_Kernel_Stats(cls, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl))
@@ -147,7 +147,7 @@ def solve_latency(ch, n, nb, reps):
     return out
 
 
-def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
+def mixed_leg(ch, n, nrhs, nb, kappas, precs=("fp32", "bf16x3", "bf16"), nrefl=16):
     """FP64 dposv against mixed precision + FP64 iterative refinement on
     conditioned SPD matrices (same A, same B, device resident)."""
     L = ch.lib
@@ -190,7 +190,7 @@ def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
                            "berr": be.value, "info": info}
             print(f"[bench] kappa={kappa:g} fp64 dposv {t64 * 1e3:.1f} ms berr {be.value:.3g}",
                   file=sys.stderr, flush=True)
-            for prec in (0, 2, 1):
+            for prec in [{"fp32": 0, "bf16x3": 2, "bf16": 1}[p] for p in precs]:
                 it, berr = ctypes.c_int(-99), ctypes.c_double(-2.0)
                 L.CHAMELEON_Enable(6)       # CHAMELEON_GENERATE_STATS
                 L.CHAMAMD_Kernel_Stats_Reset()
@@ -202,7 +202,7 @@ def mixed_leg(ch, n, nrhs, nb, kappas, nrefl=16):
                 tm = time.perf_counter() - t0
                 seq.destroy()
                 kst = {}
-                for cls, name in {0: "update", 1: "potrf", 2: "trsm", 3: "solve", 4: "panel"}.items():
+                for cls, name in {0: "update", 1: "potrf", 2: "trsm", 3: "solve", 4: "panel", 5: "colupd"}.items():
                     la, ms, fl = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
                     L.CHAMAMD_Kernel_Stats(cls, ctypes.byref(la), ctypes.byref(ms), ctypes.byref(fl))
                     kst[name] = {"launches": la.value, "ms": round(ms.value, 2),
@@ -376,6 +376,7 @@ def main():
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--kappas", default="1e2,1e4,1e6", help="condition numbers of the mixed-precision leg")
     ap.add_argument("--mixed-nb", type=int, default=0, help="tile size of the mixed-precision leg (0: the FP64 one)")
+    ap.add_argument("--mixed-precs", default="fp32,bf16x3,bf16", help="products of the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve", type=float, default=3.0, metavar="SECONDS",
                     help="arrival window of the serving leg (0: skip)")
@@ -414,7 +415,8 @@ def main():
                 "avg_launch_ms": round(up["ms"] / up["launches"], 4) if up.get("launches") else None}
     slat = None if args.no_latency else solve_latency(ch, n, nb, 30)
     mixed = None if args.no_mixed else mixed_leg(ch, n, nrhs, args.mixed_nb or nb,
-                                                 tuple(float(x) for x in args.kappas.split(",")))
+                                                 tuple(float(x) for x in args.kappas.split(",")),
+                                                 tuple(args.mixed_precs.split(",")))
     batch = None if args.no_batch else batch_leg(ch)
     serve = serve_leg(ch, args.serve) if args.serve > 0 else None
     cpu = None

@@ -68,11 +68,19 @@ void potrf_run(const PotrfPlan &p)
         (void)k;
         return f;
     };
-    auto update = [&](int k, int n0, int n1, hipStream_t s) {
-        if (n0 >= NT) return;
-        timed(CHAMAMD_KERNEL_UPDATE, update_flops(k, n0, n1), s, [&] {
-            if (p.lp_mode < 0) k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s);
-            else k::trailing_update_lp(p.A, p.P[k & 1], p.lp_mode, NT, ldt, n0, n1, NT, p.d_info, s);
+    /* paired bulk updates: the trailing matrix right of the two lookahead
+     * columns takes the panels of steps k and k+1 in one pass */
+    const bool pairs = c->pairs && c->lookahead > 0 && NT >= 4 && (p.lp_mode < 0 || p.P[3] != nullptr);
+    auto pbuf = [&](int k) { return pairs ? p.P[k % 4] : p.P[k & 1]; };
+    auto update = [&](int k, int n0, int n1, hipStream_t s, int kp = 1) {
+        if (n0 >= NT || n1 <= n0) return;
+        /* the lookahead columns (one-column launches on the critical
+         * stream) are timed apart from the bulk update */
+        const int cls = (s == sm && n1 == n0 + 1) ? CHAMAMD_KERNEL_COLUPD : CHAMAMD_KERNEL_UPDATE;
+        timed(cls, kp * update_flops(k, n0, n1), s, [&] {
+            if (p.lp_mode < 0) k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s, kp);
+            else k::trailing_update_lp(p.A, pbuf(k), kp == 2 ? pbuf(k + 1) : nullptr, p.lp_mode, NT, ldt, n0, n1, NT,
+                                       p.d_info, s);
         });
     };
     /* POTRF(k) runs next to the bulk update of step k-1 (columns k+1..).
@@ -106,7 +114,7 @@ void potrf_run(const PotrfPlan &p)
                 k::panel_factor(tile(k, k), tile(std::min(k + 1, NT - 1), k), (long long)ts, NT - k - 1, ldt, dk,
                                 flags, ctr, epoch, p.d_info, p.nb * k, nvalid, sm);
             });
-            if (k + 1 < NT && p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, p.P[k & 1], p.lp_mode, sm);
+            if (k + 1 < NT && p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf(k), p.lp_mode, sm);
             return;
         }
         const int waves = potrf_waves(k);
@@ -118,7 +126,7 @@ void potrf_run(const PotrfPlan &p)
             timed(CHAMAMD_KERNEL_TRSM, f, sm, [&] {
                 k::trsm_right_lower_trans(tile(k + 1, k), NT - k - 1, ts, ldt, tile(k, k), dk, p.d_info, sm);
             });
-            if (p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, p.P[k & 1], p.lp_mode, sm);
+            if (p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf(k), p.lp_mode, sm);
         }
     };
     if (c->lookahead <= 0 || NT <= 2) {
@@ -126,6 +134,37 @@ void potrf_run(const PotrfPlan &p)
             panel(k);
             update(k, k + 1, NT, sm);
         }
+        return;
+    }
+    if (pairs) {
+        /* Steps in pairs (k, k+1), two lookahead columns.  At the start of
+         * a pair, panel k is factored, column k+1 holds every update
+         * before step k, and the columns from k+2 on hold them once the
+         * paired bulk update of (k-2, k-1) is done.
+         *   main:   column k+1 by panel k, POTRF+TRSM(k+1);
+         *           [wait bulk(k-2,k-1)] columns k+2, k+3 by panels k, k+1;
+         *           POTRF+TRSM(k+2)
+         *   update: bulk(k, k+1) on the columns from k+4 (after panel k+1)
+         * Events: 2k = panel k+1 done, 2k+1 = bulk(k, k+1) done. */
+        CHX_HIP(hipEventRecord(event(0), sm));
+        CHX_HIP(hipStreamWaitEvent(su, event(0), 0));
+        panel(0);
+        int last_bulk = -1;
+        for (int k = 0; k + 1 < NT; k += 2) {
+            update(k, k + 1, k + 2, sm);
+            panel(k + 1);
+            if (k + 2 >= NT) break;
+            CHX_HIP(hipEventRecord(event(2 * k), sm));
+            CHX_HIP(hipStreamWaitEvent(su, event(2 * k), 0));
+            update(k, k + 4, NT, su, 2);
+            CHX_HIP(hipEventRecord(event(2 * k + 1), su));
+            if (last_bulk >= 0) CHX_HIP(hipStreamWaitEvent(sm, event(2 * last_bulk + 1), 0));
+            last_bulk = k;
+            update(k, k + 2, k + 3, sm, 2);
+            update(k, k + 3, k + 4, sm, 2);
+            panel(k + 2);
+        }
+        if (last_bulk >= 0) CHX_HIP(hipStreamWaitEvent(sm, event(2 * last_bulk + 1), 0));
         return;
     }
     /* events: P(k) = 2k (panel k done), R(k) = 2k+1 (rest update of step k

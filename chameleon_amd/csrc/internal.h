@@ -82,6 +82,7 @@ struct Context {
     int strip_epoch = 0;
     unsigned *panel_ctr = nullptr;  /* ticket / finished counters of the fused panel kernel */
     int panel = 1;                  /* fused panel kernel (CHAMAMD_PANEL=0: POTRF + TRSM)   */
+    int pairs = 1;                  /* paired bulk updates (CHAMAMD_PAIRS=0: one per step)  */
     int sched = 0;                /* lookahead issue order (CHAMAMD_SCHED) */
     long long st_launches[CHAMAMD_KERNEL_NCLASS] = {};
     double st_ms[CHAMAMD_KERNEL_NCLASS] = {};
@@ -145,7 +146,7 @@ struct PotrfPlan {
     /* mixed precision: -1 = FP64 trailing updates; 0/1/2 = FP32/BF16/BF16x3
      * operands, demoted panel alternating between P[0] and P[1] */
     int lp_mode = -1;
-    void *P[2] = {nullptr, nullptr};
+    void *P[4] = {nullptr, nullptr, nullptr, nullptr};   /* panel k in P[k % 2], or P[k % 4] with paired updates */
 };
 void potrf_run(const PotrfPlan &p);
 struct SolvePlan {
@@ -191,7 +192,8 @@ void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int l
 void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, int ldt, const double *L0,
                                     long long lstride, const double *dinv0, long long dstride, const int *info,
                                     hipStream_t s);
-void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s);
+void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
+                     int kp = 1);
 void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
 void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, const int *nvalid, double *dinv,
                 const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s);
@@ -206,8 +208,8 @@ void tile_pad_diag(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, 
                    hipStream_t s);
 /* mixed.hip */
 void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int mode, hipStream_t s);
-void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, int n0, int n1, int NT, const int *info,
-                        hipStream_t s);
+void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int lmt, int ldt, int n0, int n1, int NT,
+                        const int *info, hipStream_t s);
 void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s);
 size_t residual_scratch(int NT, int ldt, int nrp);   /* doubles of `part` */

@@ -55,6 +55,8 @@ __device__ __forceinline__ void lstore_slice(double *lds, const v2d (&r)[L], int
 
 /* ================================================================== */
 /* Trailing update of step k: A(m,n) -= A(m,k) A(n,k)^T for the tile pairs
+ * (with kp = 2, the two panels k and k+1 at once: A(m,n) -= A(m,k) A(n,k)^T
+ * + A(m,k+1) A(n,k+1)^T, one read and write of A(m,n) for both steps)
  * n <= m of either one column (n = n0, colmode) or the whole trailing
  * triangle n0 <= n <= m < NT.  Diagonal tiles (m == n) only write their
  * lower triangle: that is the SYRK of pzpotrf.c:112-117, the off-diagonal
@@ -67,7 +69,7 @@ __device__ __forceinline__ void lstore_slice(double *lds, const v2d (&r)[L], int
  * rows of one column: stores of 16 lanes form one 128-byte segment.      */
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update_kernel(
-    double *A, int lmt, int ldt, int k, int n0, int NT, int colmode, int total_blocks, const int *info)
+    double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int colmode, int total_blocks, const int *info)
 {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NTH = NW * 64;
@@ -118,25 +120,25 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
         for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = zero4d();
 
     v2d ra[LA], rb[LB];
-    const int S = ldt / BK;
-    gload_slice<BM, LA, NTH>(ra, Am, 0, ldt, tid);
-    gload_slice<BN, LB, NTH>(rb, An, 0, ldt, tid);
+    /* kp panels (tile columns k .. k+kp-1) of ldt / BK slices each */
+    const int SK = ldt / BK, S = kp * SK;
+    const size_t pstride = (size_t)lmt * ts;
+    auto gload = [&](int st) {
+        const int pi = st / SK, kb = (st - pi * SK) * BK;
+        gload_slice<BM, LA, NTH>(ra, Am + pi * pstride, kb, ldt, tid);
+        gload_slice<BN, LB, NTH>(rb, An + pi * pstride, kb, ldt, tid);
+    };
+    gload(0);
     lstore_slice<BM, LA, NTH>(As[0], ra, tid);
     lstore_slice<BN, LB, NTH>(Bs[0], rb, tid);
-    if (S > 1) {
-        gload_slice<BM, LA, NTH>(ra, Am, BK, ldt, tid);
-        gload_slice<BN, LB, NTH>(rb, An, BK, ldt, tid);
-    }
+    if (S > 1) gload(1);
     __syncthreads();
     for (int st = 0; st < S; ++st) {
         const int cur = st & 1;
         if (st + 1 < S) {
             lstore_slice<BM, LA, NTH>(As[cur ^ 1], ra, tid);
             lstore_slice<BN, LB, NTH>(Bs[cur ^ 1], rb, tid);
-            if (st + 2 < S) {
-                gload_slice<BM, LA, NTH>(ra, Am, (st + 2) * BK, ldt, tid);
-                gload_slice<BN, LB, NTH>(rb, An, (st + 2) * BK, ldt, tid);
-            }
+            if (st + 2 < S) gload(st + 2);
         }
         const double *as = As[cur], *bs = Bs[cur];
 #pragma unroll
@@ -166,7 +168,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
 }
 
 template <int BM, int BN, int WM, int WN>
-static void launch_update(double *A, int lmt, int ldt, int k, int n0, int NT, int colmode, int npairs,
+static void launch_update(double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int colmode, int npairs,
                           const int *info, hipStream_t s)
 {
     const int bpt = (ldt / BM) * (ldt / BN);
@@ -174,23 +176,24 @@ static void launch_update(double *A, int lmt, int ldt, int k, int n0, int NT, in
     if (total <= 0) return;
     constexpr int NTH = (BM / WM) * (BN / WN) * 64;
     hipLaunchKernelGGL((trailing_update_kernel<BM, BN, WM, WN>), dim3((unsigned)total), dim3(NTH), 0, s, A, lmt,
-                       ldt, k, n0, NT, colmode, (int)total, info);
+                       ldt, k, kp, n0, NT, colmode, (int)total, info);
 }
 
-void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s)
+void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
+                     int kp)
 {
     if (n0 >= NT || n1 <= n0) return;
     const bool colmode = (n1 == n0 + 1);
     const int S = NT - n0;
     const int npairs = colmode ? S : S * (S + 1) / 2;
     if (ldt % 128 == 0)
-        launch_update<128, 128, 64, 32>(A, lmt, ldt, k, n0, NT, colmode, npairs, info, s);
+        launch_update<128, 128, 64, 32>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
     else if (ldt % 64 == 0)
-        launch_update<64, 64, 32, 32>(A, lmt, ldt, k, n0, NT, colmode, npairs, info, s);
+        launch_update<64, 64, 32, 32>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
     else if (ldt % 32 == 0)
-        launch_update<32, 32, 16, 16>(A, lmt, ldt, k, n0, NT, colmode, npairs, info, s);
+        launch_update<32, 32, 16, 16>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
     else
-        launch_update<16, 16, 16, 16>(A, lmt, ldt, k, n0, NT, colmode, npairs, info, s);
+        launch_update<16, 16, 16, 16>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
 }
 
 /* ================================================================== */

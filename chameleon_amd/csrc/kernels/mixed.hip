@@ -105,7 +105,8 @@ __device__ __forceinline__ void decode_pair(int pair, int n0, int colmode, int &
  * wavefronts of 64 x 32, K in 32-deep LDS slices (double-buffered). */
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kernel(
-    double *A, const float *P, int lmt, int ldt, int n0, int colmode, int total_blocks, const int *info)
+    double *A, const float *P, const float *P2, int lmt, int ldt, int n0, int colmode, int total_blocks,
+    const int *info)
 {
     constexpr int NW = (BM / WM) * (BN / WN), NTH = NW * 64, BK = 32;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -133,16 +134,21 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = zero4f();
     v4f ra[LA], rb[LB];
-    auto gload = [&](int kb) {
+    /* slices of the first panel, then (P2 given) of the second */
+    const int SK = ldt / BK;
+    const ptrdiff_t d2 = P2 ? P2 - P : 0;
+    auto gload = [&](int st) {
+        const int pi = st / SK, kb = (st - pi * SK) * BK;
+        const float *am = Am + pi * d2, *an = An + pi * d2;
 #pragma unroll
         for (int it = 0; it < LA; ++it) {
             const int e = (it * NTH + tid) * 4;
-            ra[it] = *reinterpret_cast<const v4f *>(Am + (e % BM) + (size_t)(kb + e / BM) * ldt);
+            ra[it] = *reinterpret_cast<const v4f *>(am + (e % BM) + (size_t)(kb + e / BM) * ldt);
         }
 #pragma unroll
         for (int it = 0; it < LB; ++it) {
             const int e = (it * NTH + tid) * 4;
-            rb[it] = *reinterpret_cast<const v4f *>(An + (e % BN) + (size_t)(kb + e / BN) * ldt);
+            rb[it] = *reinterpret_cast<const v4f *>(an + (e % BN) + (size_t)(kb + e / BN) * ldt);
         }
     };
     auto lstore = [&](int buf) {
@@ -151,16 +157,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
 #pragma unroll
         for (int it = 0; it < LB; ++it) *reinterpret_cast<v4f *>(&Bs[buf][(it * NTH + tid) * 4]) = rb[it];
     };
-    const int S = ldt / BK;
+    const int S = (P2 ? 2 : 1) * SK;
     gload(0);
     lstore(0);
-    if (S > 1) gload(BK);
+    if (S > 1) gload(1);
     __syncthreads();
     for (int st = 0; st < S; ++st) {
         const int cur = st & 1;
         if (st + 1 < S) {
             lstore(cur ^ 1);
-            if (st + 2 < S) gload((st + 2) * BK);
+            if (st + 2 < S) gload(st + 2);
         }
         const float *as = As[cur], *bs = Bs[cur];
 #pragma unroll
@@ -196,9 +202,9 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
  * lane of a 16x16x32 MFMA supplies 8 consecutive k of one row (the same k
  * assignment on both operands, so the dot product is exact in any order). */
 template <int SPLIT>
-__global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const unsigned short *P, int NT, int lmt,
-                                                             int ldt, int n0, int colmode, int total_blocks,
-                                                             const int *info)
+__global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const unsigned short *P,
+                                                             const unsigned short *P2, int NT, int lmt, int ldt,
+                                                             int n0, int colmode, int total_blocks, const int *info)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 64, LDK = BK + 8;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -224,11 +230,14 @@ __global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const un
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = zero4f();
     /* slice loader: BM rows x BK k of each plane = BM*BK/8 v8 loads = 2 per thread per operand plane */
-    for (int kb = 0; kb < ldt; kb += BK) {
+    const int K = (P2 ? 2 : 1) * ldt;      /* the second panel (P2) follows the first */
+    for (int kb2 = 0; kb2 < K; kb2 += BK) {
+        const unsigned short *Pp = kb2 < ldt ? P : P2;
+        const int kb = kb2 < ldt ? kb2 : kb2 - ldt;
 #pragma unroll
         for (int pl = 0; pl < NP; ++pl) {
-            const unsigned short *Pm = P + ((size_t)pl * NT + m) * ts;
-            const unsigned short *Pn = P + ((size_t)pl * NT + n) * ts;
+            const unsigned short *Pm = Pp + ((size_t)pl * NT + m) * ts;
+            const unsigned short *Pn = Pp + ((size_t)pl * NT + n) * ts;
 #pragma unroll
             for (int it = 0; it < (BM * BK) / (8 * NTH); ++it) {
                 const int e = (it * NTH + tid) * 8;
@@ -256,13 +265,20 @@ __global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const un
 #pragma unroll
             for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn) {
+                for (int fn = 0; fn < FN; ++fn)
                     acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[fn], ah[fm], acc[fm][fn], 0, 0, 0);
-                    if (SPLIT == 3) {
+            if (SPLIT == 3) {
+#pragma unroll
+                for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                    for (int fn = 0; fn < FN; ++fn)
                         acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[fn], ah[fm], acc[fm][fn], 0, 0, 0);
+#pragma unroll
+                for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                    for (int fn = 0; fn < FN; ++fn)
                         acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[fn], al[fm], acc[fm][fn], 0, 0, 0);
-                    }
-                }
+            }
         }
         __syncthreads();
     }
@@ -278,8 +294,124 @@ __global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const un
             }
 }
 
-void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, int n0, int n1, int NT, const int *info,
-                        hipStream_t s)
+/* Pipelined form of the BF16-operand update: 32-deep k slices, two LDS
+ * stages (stage s+1 is stored while stage s is multiplied) and global
+ * loads issued two slices ahead, one barrier per slice.  LDS rows are the
+ * 32 k of one row (64 bytes), so the 16 rows x 4 k-groups a wavefront reads
+ * per operand are one contiguous 1 KiB block. */
+template <int SPLIT>
+__global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const unsigned short *P,
+                                                              const unsigned short *P2, int NT, int lmt, int ldt,
+                                                              int n0, int colmode, int total_blocks, const int *info)
+{
+    constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 32;
+    constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
+    constexpr int NP = (SPLIT == 3) ? 2 : 1;          /* planes per operand */
+    __shared__ unsigned short As[2][NP][BM * BK];
+    __shared__ unsigned short Bs[2][NP][BN * BK];
+    if (*info != 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int bid = xcd_remap_m(blockIdx.x, total_blocks);
+    const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
+    int m, n;
+    decode_pair(bid / bpt, n0, colmode, m, n);
+    const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
+    const bool diag = (m == n);
+    if (diag && col0 >= row0 + BM) return;
+    const size_t ts = (size_t)ldt * ldt;
+    double *C = A + ((size_t)n * lmt + m) * ts;
+    const int wm = (wave % (BM / WM)) * WM, wn = (wave / (BM / WM)) * WN;
+    v4f acc[FM][FN];
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = zero4f();
+    /* one 16-byte piece of each operand plane per thread and slice:
+     * row tid / 4, k 8 * (tid % 4) */
+    const int lr = tid >> 2, lq = (tid & 3) * 8;
+    const unsigned short *pa[NP], *pb[NP];
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        pa[pl] = P + ((size_t)pl * NT + m) * ts + (size_t)(row0 + lr) * ldt + lq;
+        pb[pl] = P + ((size_t)pl * NT + n) * ts + (size_t)(col0 + lr) * ldt + lq;
+    }
+    v8u16 ra[NP], rb[NP];
+    /* slices of the first panel, then (P2 given) of the second */
+    const int SK = ldt / BK;
+    const ptrdiff_t d2 = P2 ? P2 - P : 0;
+    auto gload = [&](int st) {
+        const int pi = st / SK, kb = (st - pi * SK) * BK;
+#pragma unroll
+        for (int pl = 0; pl < NP; ++pl) {
+            ra[pl] = *reinterpret_cast<const v8u16 *>(pa[pl] + pi * d2 + kb);
+            rb[pl] = *reinterpret_cast<const v8u16 *>(pb[pl] + pi * d2 + kb);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int pl = 0; pl < NP; ++pl) {
+            *reinterpret_cast<v8u16 *>(&As[buf][pl][lr * BK + lq]) = ra[pl];
+            *reinterpret_cast<v8u16 *>(&Bs[buf][pl][lr * BK + lq]) = rb[pl];
+        }
+    };
+    const int S = (P2 ? 2 : 1) * SK;
+    gload(0);
+    lstore(0);
+    if (S > 1) gload(1);
+    __syncthreads();
+    for (int st = 0; st < S; ++st) {
+        const int cur = st & 1;
+        if (st + 1 < S) {
+            lstore(cur ^ 1);
+            if (st + 2 < S) gload(st + 2);
+        }
+        v8bf ah[FM], bh[FN], al[FM], bl[FN];
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm) {
+            ah[fm] = *reinterpret_cast<const v8bf *>(&As[cur][0][(wm + fm * 16 + li) * BK + 8 * lk]);
+            if (SPLIT == 3) al[fm] = *reinterpret_cast<const v8bf *>(&As[cur][NP - 1][(wm + fm * 16 + li) * BK + 8 * lk]);
+        }
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) {
+            bh[fn] = *reinterpret_cast<const v8bf *>(&Bs[cur][0][(wn + fn * 16 + li) * BK + 8 * lk]);
+            if (SPLIT == 3) bl[fn] = *reinterpret_cast<const v8bf *>(&Bs[cur][NP - 1][(wn + fn * 16 + li) * BK + 8 * lk]);
+        }
+        /* the three products of one accumulator are FM*FN products apart,
+         * so no MFMA waits for the result of the one just before it */
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+                acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[fn], ah[fm], acc[fm][fn], 0, 0, 0);
+        if (SPLIT == 3) {
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[fn], ah[fm], acc[fm][fn], 0, 0, 0);
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn)
+                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[fn], al[fm], acc[fm][fn], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int gi = row0 + wm + fm * 16 + li;
+                const int gj = col0 + wn + fn * 16 + 4 * lk + s;
+                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= (double)acc[fm][fn][s];
+            }
+}
+
+void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int lmt, int ldt, int n0, int n1, int NT,
+                        const int *info, hipStream_t s)
 {
     if (n0 >= NT || n1 <= n0) return;
     const bool colmode = (n1 == n0 + 1);
@@ -290,13 +422,17 @@ void trailing_update_lp(double *A, const void *P, int mode, int lmt, int ldt, in
     if (total <= 0) return;
     if (mode == 0)
         hipLaunchKernelGGL((update_f32_kernel<128, 128, 64, 32>), dim3((unsigned)total), dim3(512), 0, s, A,
-                           (const float *)P, lmt, ldt, n0, (int)colmode, (int)total, info);
-    else if (mode == 1)
-        hipLaunchKernelGGL((update_bf16_kernel<1>), dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
-                           NT, lmt, ldt, n0, (int)colmode, (int)total, info);
-    else
-        hipLaunchKernelGGL((update_bf16_kernel<3>), dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
-                           NT, lmt, ldt, n0, (int)colmode, (int)total, info);
+                           (const float *)P, (const float *)P2, lmt, ldt, n0, (int)colmode, (int)total, info);
+    else {
+        static const int pipelined = [] {
+            const char *e = getenv("CHAMAMD_LP_PIPE");
+            return e ? atoi(e) : 0;
+        }();
+        auto *kern = pipelined ? (mode == 1 ? update_bf16p_kernel<1> : update_bf16p_kernel<3>)
+                               : (mode == 1 ? update_bf16_kernel<1> : update_bf16_kernel<3>);
+        hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
+                           (const unsigned short *)P2, NT, lmt, ldt, n0, (int)colmode, (int)total, info);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -316,13 +452,15 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
                                                         double *part)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int rows = NT * ldt;
     const int jchunks = (nrp + 63) / 64;
     const int nrb = rows / 16;
-    const int ks = gw % RES_KS, rest = gw / RES_KS;
-    const int rb = rest % nrb, jc = rest / nrb;
-    if (jc >= jchunks) return;
+    /* the four waves of a workgroup take four consecutive 16-row blocks of
+     * one k range, so they read the same X chunks at the same time */
+    const int groups = (nrb + 3) / 4;
+    const int ks = blockIdx.x % RES_KS, rest = blockIdx.x / RES_KS;
+    const int rb = (rest % groups) * 4 + (threadIdx.x >> 6), jc = rest / groups;
+    if (jc >= jchunks || rb >= nrb) return;     /* no barriers below: waves are independent */
     const int r0 = rb * 16;
     const int m = r0 / ldt, i0 = r0 % ldt;
     const int j0 = jc * 64;
@@ -444,8 +582,8 @@ size_t residual_scratch(int NT, int ldt, int nrp) { return (size_t)RES_KS * NT *
 void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s)
 {
-    const long long waves = (long long)(NT * ldt / 16) * ((nrp + 63) / 64) * RES_KS;
-    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
+    const long long blocks = (long long)((NT * ldt / 16 + 3) / 4) * ((nrp + 63) / 64) * RES_KS;
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
     const size_t n = (size_t)NT * ldt * nrp;
     hipLaunchKernelGGL(residual_sum_kernel, dim3(2048), dim3(256), 0, s, BT, part, RT, n);
 }

@@ -152,15 +152,14 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
     if (lp_ok) {
         const double anrm = norm_inf_sym(A);
         const double cte = anrm * eps * std::sqrt((double)N);
-        /* workspace2: working factor | panels (2) | BT | XT | RT | norms | residual pieces */
+        /* workspace2: working factor | panels (4) | BT | XT | RT | norms | residual pieces */
         const size_t abytes = MA->bytes;
         const size_t pbytes = (size_t)NT * ts * 4;
         const size_t vb = (size_t)nrp * rows;
         const size_t pv = k::residual_scratch(NT, ldt, nrp);
-        char *w = (char *)workspace2(abytes + 2 * pbytes + (3 * vb + nrp + pv) * sizeof(double));
+        char *w = (char *)workspace2(abytes + 4 * pbytes + (3 * vb + nrp + pv) * sizeof(double));
         double *Aw = (double *)w;
-        void *P0 = w + abytes, *P1 = w + abytes + pbytes;
-        double *BT = (double *)(w + abytes + 2 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb, *part = nw + nrp;
+        double *BT = (double *)(w + abytes + 4 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb, *part = nw + nrp;
         CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
 
         PotrfPlan p;
@@ -171,8 +170,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         p.nvalid_last = N - (NT - 1) * A->mb;
         p.d_info = st->d_info;
         p.lp_mode = prec;
-        p.P[0] = P0;
-        p.P[1] = P1;
+        for (int i = 0; i < 4; ++i) p.P[i] = w + abytes + i * pbytes;
         potrf_run(p);
         st->pending_info = true;
         const int info = sequence_wait_internal(seq);

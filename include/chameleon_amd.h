@@ -17,12 +17,13 @@ extern "C" {
  * runtime/starpu/control/runtime_profiling.c:120-150; here each launch of
  * the class is bracketed by HIP events on its own stream). */
 enum {
-    CHAMAMD_KERNEL_UPDATE = 0,   /* trailing SYRK+GEMM update of the factorisation */
+    CHAMAMD_KERNEL_UPDATE = 0,   /* bulk trailing SYRK+GEMM update of the factorisation */
     CHAMAMD_KERNEL_POTRF  = 1,   /* diagonal tile POTRF                             */
     CHAMAMD_KERNEL_TRSM   = 2,   /* panel TRSM                                      */
     CHAMAMD_KERNEL_SOLVE  = 3,   /* triangular-solve GEMMs (forward + backward)     */
     CHAMAMD_KERNEL_PANEL  = 4,   /* fused diagonal POTRF + panel TRSM of one step   */
-    CHAMAMD_KERNEL_NCLASS = 5
+    CHAMAMD_KERNEL_COLUPD = 5,   /* lookahead column updates (the critical stream)  */
+    CHAMAMD_KERNEL_NCLASS = 6
 };
 
 /* Accumulated launches, summed device time (ms) and algorithmic flops

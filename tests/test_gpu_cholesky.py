@@ -319,7 +319,11 @@ def test_async_device_resident_matches_sync(ch):
     assert np.array_equal(X, B)
 
 
-def test_lookahead_does_not_change_bits(ch):
+def test_lookahead_schedules_agree(ch):
+    """With lookahead the bulk update takes two panels per pass (one read and
+    write of each trailing tile for two steps), which changes the order in
+    which the two rank-nb terms are added: the factors of the two schedules
+    agree to rounding and both pass the factorisation check."""
     n, nb = 1500, 128
     say("lookahead", n, nb)
     ch.set_param(ch.TILE_SIZE, nb)
@@ -330,7 +334,10 @@ def test_lookahead_does_not_change_bits(ch):
         assert ch.lib.CHAMELEON_dpotrf(L_, n, _p(A), n) == 0
         out.append(np.tril(A))
     ch.set_param(ch.LOOKAHEAD, 1)
-    assert np.array_equal(out[0], out[1])
+    assert oracle.rel_frobenius(out[0], out[1]) <= 1e-14
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    for F in out:
+        assert oracle.check_factorization(A0, F, L_)[0]
 
 
 def test_upper_is_transpose_of_lower(ch):
