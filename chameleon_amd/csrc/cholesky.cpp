@@ -168,40 +168,22 @@ void potrf_run(const PotrfPlan &p)
         return;
     }
     /* events: P(k) = 2k (panel k done), R(k) = 2k+1 (rest update of step k
-     * done), C(k) = 2NT+k (column k updated by step k-1) */
+     * done) */
     /* the update stream must not start before earlier work on the main stream */
     CHX_HIP(hipEventRecord(event(0), sm));
     CHX_HIP(hipStreamWaitEvent(su, event(0), 0));
     panel(0);
     CHX_HIP(hipEventRecord(event(2 * 0), sm));
-    if (c->sched == 1) {
-        /* The bulk update of step k is held back until column k+1 is
-         * updated, so that it is released together with POTRF(k+1): the
-         * one-workgroup POTRF then reaches an idle compute unit before the
-         * bulk grid fills the device (a POTRF queued behind a running
-         * bulk grid only starts once that grid drains). */
-        for (int k = 0; k < NT - 1; ++k) {
-            if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
-            update(k, k + 1, k + 2, sm);
-            CHX_HIP(hipEventRecord(event(2 * NT + k + 1), sm));
-            panel(k + 1);
-            CHX_HIP(hipEventRecord(event(2 * (k + 1)), sm));
-            CHX_HIP(hipStreamWaitEvent(su, event(2 * NT + k + 1), 0));
-            if (k + 2 < NT) update(k, k + 2, NT, su);
-            CHX_HIP(hipEventRecord(event(2 * k + 1), su));
-        }
-    } else {
-        for (int k = 0; k < NT - 1; ++k) {
-            /* update stream: columns k+2.. with panel k */
-            CHX_HIP(hipStreamWaitEvent(su, event(2 * k), 0));
-            if (k + 2 < NT) update(k, k + 2, NT, su);
-            CHX_HIP(hipEventRecord(event(2 * k + 1), su));
-            /* critical stream: column k+1, then panel k+1 */
-            if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
-            update(k, k + 1, k + 2, sm);
-            panel(k + 1);
-            CHX_HIP(hipEventRecord(event(2 * (k + 1)), sm));
-        }
+    for (int k = 0; k < NT - 1; ++k) {
+        /* update stream: columns k+2.. with panel k */
+        CHX_HIP(hipStreamWaitEvent(su, event(2 * k), 0));
+        if (k + 2 < NT) update(k, k + 2, NT, su);
+        CHX_HIP(hipEventRecord(event(2 * k + 1), su));
+        /* critical stream: column k+1, then panel k+1 */
+        if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
+        update(k, k + 1, k + 2, sm);
+        panel(k + 1);
+        CHX_HIP(hipEventRecord(event(2 * (k + 1)), sm));
     }
     /* join: the last rest update ends before anything else on the main stream */
     CHX_HIP(hipStreamWaitEvent(sm, event(2 * (NT - 2) + 1), 0));

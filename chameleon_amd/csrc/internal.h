@@ -83,7 +83,6 @@ struct Context {
     unsigned *panel_ctr = nullptr;  /* ticket / finished counters of the fused panel kernel */
     int panel = 1;                  /* fused panel kernel (CHAMAMD_PANEL=0: POTRF + TRSM)   */
     int pairs = 1;                  /* paired bulk updates (CHAMAMD_PAIRS=0: one per step)  */
-    int sched = 0;                /* lookahead issue order (CHAMAMD_SCHED) */
     long long st_launches[CHAMAMD_KERNEL_NCLASS] = {};
     double st_ms[CHAMAMD_KERNEL_NCLASS] = {};
     double st_flops[CHAMAMD_KERNEL_NCLASS] = {};

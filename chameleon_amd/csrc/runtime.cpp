@@ -340,7 +340,6 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     c->ib = env_int("CHAMELEON_INNER_BLOCK_SIZE", 48);
     c->lookahead = env_int("CHAMELEON_LOOKAHEAD", 1);
     c->translation = env_int("CHAMELEON_TRANSLATION_MODE", ChamInPlace);
-    c->sched = env_int("CHAMAMD_SCHED", 0);
     c->panel = env_int("CHAMAMD_PANEL", 1);
     c->pairs = env_int("CHAMAMD_PAIRS", 1);
     CHX_HIP(hipSetDevice(c->device));
