@@ -234,7 +234,8 @@ def test_non_spd_info_fused_panel(ch, nb, uplo):
 
 # ---------------------------------------------------------- large tile sizes
 @pytest.mark.parametrize("n,nb,nrhs", [(1000, 128, 16), (1000, 320, 64), (2048, 256, 64), (2100, 384, 64),
-                                       (2048, 512, 1), (1536, 192, 3)])
+                                       (2048, 512, 1), (1536, 192, 3), (1100, 128, 5), (2250, 256, 64),
+                                       (2600, 512, 17)])
 @pytest.mark.parametrize("uplo", [L_, U_])
 def test_large_tiles_vs_oracle(ch, n, nb, nrhs, uplo):
     say("large", n, nb, nrhs, uplo)
