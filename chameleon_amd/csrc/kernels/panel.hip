@@ -116,6 +116,30 @@ __device__ __forceinline__ void block16_potrf(double (&a)[16], int lane, int &fa
     }
 }
 
+/* One 16-column step of a 64-row column panel by one wavefront: lanes
+ * 0-15 hold the rows of the 16 x 16 diagonal block, lanes 16-63 the rows
+ * below it, a[] their 16 columns.  The right-looking Cholesky steps that
+ * factor the diagonal block (pivots and column entries read from lanes
+ * 0-15) turn the rows below into X = P L16^-T at the same time.  Entries
+ * above the diagonal of lanes 0-15 end up meaningless (never read).  rd =
+ * 1 / L(l,l) on lanes l < 16; `fail` as in block16_potrf. */
+__device__ __forceinline__ void block16_panel(double (&a)[16], int lane, int &fail, double &rd)
+{
+    fail = 0;
+    rd = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double d = readlane_f64(a[j], j);
+        if (fail == 0 && !(d > 0.0)) fail = j + 1;
+        const double r = rsqrt_f64(d);
+        const double lij = a[j] * r;
+        if (lane == j) rd = r;
+        a[j] = lij;
+#pragma unroll
+        for (int k = j + 1; k < 16; ++k) a[k] -= lij * readlane_f64(lij, k);
+    }
+}
+
 /* Column (lane % 16) of the inverse of the lower triangular 16 x 16 block
  * L(r,p) = Ls[r * lds + p], by forward substitution, with the reciprocal
  * diagonal rdiag[r] = 1 / L(r,r). */
@@ -717,9 +741,14 @@ __device__ void pf_kloop(const double *X, const double *Y, int ldt, int n, doubl
 }
 
 /* Factor the 64 x 64 block in Ld (lower part, zeros above) in place and
- * build its inverse in Lv: the 16-column steps of potrf_jobs_kernel's (b)
- * and (c) phases for 4 wavefronts.  Returns the 1-based column of the
- * first non-positive pivot, 0 if none (uniform). */
+ * build its inverse in Lv, by 16-column steps: (1) columns [c0, c0+16)
+ * of rows [c0, 64) take the earlier steps' products (MFMA), (2) one
+ * wavefront factors that 16-column panel in registers (block16_panel:
+ * the diagonal block and, in the same steps, the rows below).  The
+ * 16 x 16 inverses are only needed for the 64 x 64 inverse, so the four
+ * are formed after the loop, one per wavefront, off the column-to-column
+ * chain.  Rd holds 1 / L(i,i) of all 64 columns.  Returns the 1-based
+ * column of the first non-positive pivot, 0 if none (uniform). */
 __device__ int pf_factor64(double (*Ld)[OBP], double (*Lb)[16][17], double (*Lv)[OBP], double *Rd, int *s_fail,
                            int wave, int lane)
 {
@@ -744,46 +773,38 @@ __device__ int pf_factor64(double (*Ld)[OBP], double (*Lb)[16][17], double (*Lv)
             __syncthreads();
         }
         if (wave == 0) {
+            /* lane (g, i): row c0 + 16 g + i of the panel, g = 0 the diagonal block */
+            const int row = c0 + 16 * lk + li;
+            const bool valid = row < OB;
             double a[16];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) a[q] = (q <= li) ? Ld[c0 + li][c0 + q] : 0.0;
+            for (int q = 0; q < 16; ++q) a[q] = (valid && (lk > 0 || q <= li)) ? Ld[row][c0 + q] : 0.0;
             int fail;
             double rd;
-            block16_potrf(a, lane, fail, rd);
-            if (lane < 16) {
+            block16_panel(a, lane, fail, rd);
+            if (lk == 0) {
 #pragma unroll
-                for (int q = 0; q < 16; ++q) Ld[c0 + li][c0 + q] = (q <= li) ? a[q] : 0.0;
-                Rd[li] = rd;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            double col[16];
-            block16_inverse_col(&Ld[c0][c0], OBP, Rd, lane, col);
-            if (lane < 16) {
+                for (int q = 0; q < 16; ++q) Ld[row][c0 + q] = (q <= li) ? a[q] : 0.0;
+                Rd[c0 + li] = rd;
+            } else if (valid) {
 #pragma unroll
-                for (int q = 0; q < 16; ++q) Lb[cb][q][li] = col[q];
+                for (int q = 0; q < 16; ++q) Ld[row][c0 + q] = a[q];
             }
             if (lane == 0 && fail && *s_fail == 0) *s_fail = c0 + fail;
         }
         __syncthreads();
         if (*s_fail) return *s_fail;
-        {
-            const int rb = wave;
-            if (rb < 4 - cb - 1) {
-                const int r0 = c0 + 16 + 16 * rb;
-                double pv[4];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) pv[kk] = Ld[r0 + li][c0 + 4 * kk + lk];
-                v4d acc = zero4d();
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) acc = mfma_f64(Lb[cb][li][4 * kk + lk], pv[kk], acc);
-#pragma unroll
-                for (int s = 0; s < 4; ++s) Ld[r0 + li][c0 + lk + 4 * s] = acc[s];
-            }
-        }
-        __syncthreads();
     }
+    /* the four 16 x 16 inverses, one per wavefront */
+    {
+        double col[16];
+        block16_inverse_col(&Ld[16 * wave][16 * wave], OBP, &Rd[16 * wave], lane, col);
+        if (lane < 16) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) Lb[wave][q][li] = col[q];
+        }
+    }
+    __syncthreads();
     assemble_inverse(Ld, Lb, Lv, 4, wave, lane);
     return 0;
 }
@@ -801,23 +822,14 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
     const int rowo = 16 * wave + li;
     const rsrc_t rR = pf_rsrc(R);
     const int vr = (rowo + lk * ldt) * 8;          /* R(rowo, lk) */
-    /* owner r keeps A(r, r) - sum_{p<r-1} X(r, p) X(r, p)^T of its diagonal
-     * block in Ld ([row][col]) and the last X block in Ys ([k][row]) */
+    /* owner r keeps A(r, r) - sum_{p<=C} X(r, p) X(r, p)^T of its diagonal
+     * block in Ld ([row][col]): each X block just published is folded in
+     * while the next one is awaited */
     for (int C = 0; C < nsteps; ++C) {
         v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
         if (C > 0) {
             if (!pf_wait(a, C * ND + C - 1, tid, s_int)) return;
             pf_kloop(R, a.T + 64 * C, ldt, C, Ys, tid, wave, lane, acc);
-        }
-        if (owner && C == r - 1) {
-            /* SYRK part of the diagonal block over p < r-1, before the wait */
-            v4d accd[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
-            if (C > 0) pf_kloop(R, R, ldt, C, Ys, tid, wave, lane, accd);
-#pragma unroll
-            for (int f = 0; f < 4; ++f)
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    Ld[rowo][16 * f + lk + 4 * s] = pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8) - accd[f][s];
         }
         double av[4][4];
 #pragma unroll
@@ -846,35 +858,41 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
             for (int s = 0; s < 4; ++s) pf_st(xo[f][s], rR, vr, (64 * C + 16 * f + 4 * s) * ldt * 8);
         if (owner) {
             pf_publish(a, r * ND + C, tid);
-            if (C == r - 1) {
-                /* the last X block, [k][row], for the rank-64 term */
+            /* rank-64 term X(r, C) X(r, C)^T of the diagonal block: the block
+             * through LDS ([k][row]) as the shared operand */
 #pragma unroll
-                for (int f = 0; f < 4; ++f)
+            for (int f = 0; f < 4; ++f)
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) Ys[(16 * f + lk + 4 * s) * 64 + rowo] = xo[f][s];
+                for (int s = 0; s < 4; ++s) Ys[(16 * f + lk + 4 * s) * 64 + rowo] = xo[f][s];
+            __syncthreads();
+            v4d a2[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double xq = xo[q >> 2][q & 3];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) a2[f] = mfma_f64(Ys[(4 * q + lk) * 64 + 16 * f + li], xq, a2[f]);
             }
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int c = 16 * f + lk + 4 * s;
+                    const double base = C == 0 ? pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8) : Ld[rowo][c];
+                    Ld[rowo][c] = base - a2[f][s];
+                }
         }
     }
     if (!owner) return;
 
-    /* diagonal block of owner r: A(r, r) - sum_p X(r, p) X(r, p)^T */
+    /* diagonal block of owner r: A(r, r) - sum_p X(r, p) X(r, p)^T, lower part */
     double (*Lv)[OBP] = reinterpret_cast<double (*)[OBP]>(Ys);
-    v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
     __syncthreads();
-    if (r > 0) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double xq = Ys[(4 * q + lk) * 64 + rowo];
-#pragma unroll
-            for (int f = 0; f < 4; ++f) acc[f] = mfma_f64(Ys[(4 * q + lk) * 64 + 16 * f + li], xq, acc[f]);
-        }
-    }
 #pragma unroll
     for (int f = 0; f < 4; ++f)
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int c = 16 * f + lk + 4 * s;
-            const double v = (r > 0 ? Ld[rowo][c] : pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8)) - acc[f][s];
+            const double v = r > 0 ? Ld[rowo][c] : pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8);
             Ld[rowo][c] = (rowo >= c) ? v : 0.0;
         }
     __syncthreads();
@@ -902,7 +920,7 @@ __global__ __launch_bounds__(256, 2) void panel_factor_kernel(PanelArgs a)
     __shared__ double Ys[OB * OBP];       /* Y blocks, Linv_C, the inverse being built */
     __shared__ double Ld[OB][OBP];        /* diagonal block, the last X block          */
     __shared__ double Lb[4][16][17];
-    __shared__ double Rd[16];
+    __shared__ double Rd[64];
     __shared__ int s_int, s_ticket, s_info;
     const int tid = threadIdx.x;
     __builtin_amdgcn_s_setprio(2);        /* critical path: win issue next to update waves */
