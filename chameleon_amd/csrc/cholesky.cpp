@@ -6,23 +6,27 @@
  * For each panel k the reference submits POTRF(k,k), TRSM(m,k) for m > k,
  * SYRK(n,n) and GEMM(m,n) for k < n <= m, and a runtime (StarPU, OpenMP)
  * orders them by data dependencies.  Here the same right-looking order is
- * issued as three kinds of launch per step on two HIP streams:
+ * issued as launches on two HIP streams of default priority:
  *
- *   critical stream  : update column k+1 with panel k   (trailing_update, colmode)
- *                      POTRF of tile (k+1,k+1)           (potrf_diag, 1 workgroup)
- *                      TRSM of tiles (m,k+1), m > k+1     (trsm_right, 1 wave / 16 rows)
- *   update stream    : update columns k+2..NT-1 with panel k  (trailing_update, triangle)
+ *   critical stream : the panel of step k (POTRF of A(k,k) and TRSM of the
+ *                     tiles below in one panel_factor launch) and the
+ *                     lookahead columns (trailing_update on ncol columns)
+ *   update stream   : the bulk of the trailing update (trailing_update on
+ *                     the triangle right of the lookahead columns)
  *
- * with events so that panel k+1 is factored while the bulk of step k's
- * trailing update still runs (lookahead of one panel, the role of the task
- * priorities 2NT-2k[-m][-n] in pzpotrf.c:90-123).  With
- * CHAMELEON_LOOKAHEAD = 0 everything is issued on one stream.  The Upper
- * variant runs on the transposed device mirror (U = L^T).
+ * By default the steps go in pairs (k, k+1) with two lookahead columns and
+ * one bulk launch that applies both panels (potrf_run); events order the
+ * streams, so the next panels are factored while the bulk update still
+ * runs (the role of the task priorities 2NT-2k[-m][-n] in
+ * pzpotrf.c:90-123).  CHAMELEON_LOOKAHEAD = 0 issues everything on one
+ * stream.  The Upper variant runs on the transposed device mirror (U = L^T).
  *
- * Solves (chameleon_pztrsm, compute/pztrsm.c:134-201): the inverse of every
- * diagonal tile is formed once (trsm_right on identity tiles), then each
- * step k multiplies the solved block by it and updates the blocks below
- * (forward) or above (backward) in one strided batched GEMM launch.
+ * Solves (chameleon_pztrsm, compute/pztrsm.c:134-201): with tiles a
+ * multiple of 64, the 64 x 64 diagonal-block inverses are formed once and
+ * one strip-solve launch per direction walks the 64-row strips
+ * (kernels/solve.hip); other tile sizes multiply by tile inverses and
+ * update the blocks below (forward) or above (backward) in one strided
+ * batched GEMM launch per step.
  */
 #include "internal.h"
 #include "chameleon_amd.h"
@@ -73,10 +77,11 @@ void potrf_run(const PotrfPlan &p)
     const bool pairs = c->pairs && c->lookahead > 0 && NT >= 4 && (p.lp_mode < 0 || p.P[3] != nullptr);
     auto pbuf = [&](int k) { return pairs ? p.P[k % 4] : p.P[k & 1]; };
     auto update = [&](int k, int n0, int n1, hipStream_t s, int kp = 1) {
+        n1 = std::min(n1, NT);
         if (n0 >= NT || n1 <= n0) return;
-        /* the lookahead columns (one-column launches on the critical
-         * stream) are timed apart from the bulk update */
-        const int cls = (s == sm && n1 == n0 + 1) ? CHAMAMD_KERNEL_COLUPD : CHAMAMD_KERNEL_UPDATE;
+        /* with lookahead, the critical stream only updates the lookahead
+         * columns; they are timed apart from the bulk update */
+        const int cls = (s == sm && c->lookahead > 0) ? CHAMAMD_KERNEL_COLUPD : CHAMAMD_KERNEL_UPDATE;
         timed(cls, kp * update_flops(k, n0, n1), s, [&] {
             if (p.lp_mode < 0) k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s, kp);
             else k::trailing_update_lp(p.A, pbuf(k), kp == 2 ? pbuf(k + 1) : nullptr, p.lp_mode, NT, ldt, n0, n1, NT,
@@ -142,7 +147,8 @@ void potrf_run(const PotrfPlan &p)
          * before step k, and the columns from k+2 on hold them once the
          * paired bulk update of (k-2, k-1) is done.
          *   main:   column k+1 by panel k, POTRF+TRSM(k+1);
-         *           [wait bulk(k-2,k-1)] columns k+2, k+3 by panels k, k+1;
+         *           [wait bulk(k-2,k-1)] columns k+2, k+3 by panels k, k+1
+         *           (one two-column launch);
          *           POTRF+TRSM(k+2)
          *   update: bulk(k, k+1) on the columns from k+4 (after panel k+1)
          * Events: 2k = panel k+1 done, 2k+1 = bulk(k, k+1) done. */
@@ -158,10 +164,10 @@ void potrf_run(const PotrfPlan &p)
             CHX_HIP(hipStreamWaitEvent(su, event(2 * k), 0));
             update(k, k + 4, NT, su, 2);
             CHX_HIP(hipEventRecord(event(2 * k + 1), su));
+            if (p.after_panels) p.after_panels(p.after_arg, k + 1);
             if (last_bulk >= 0) CHX_HIP(hipStreamWaitEvent(sm, event(2 * last_bulk + 1), 0));
             last_bulk = k;
-            update(k, k + 2, k + 3, sm, 2);
-            update(k, k + 3, k + 4, sm, 2);
+            update(k, k + 2, k + 4, sm, 2);
             panel(k + 2);
         }
         if (last_bulk >= 0) CHX_HIP(hipStreamWaitEvent(sm, event(2 * last_bulk + 1), 0));
@@ -180,6 +186,7 @@ void potrf_run(const PotrfPlan &p)
         if (k + 2 < NT) update(k, k + 2, NT, su);
         CHX_HIP(hipEventRecord(event(2 * k + 1), su));
         /* critical stream: column k+1, then panel k+1 */
+        if (p.after_panels) p.after_panels(p.after_arg, k);
         if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
         update(k, k + 1, k + 2, sm);
         panel(k + 1);
@@ -351,9 +358,141 @@ int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequen
     return CHAMELEON_SUCCESS;
 }
 
+/* dposv with the forward substitution beside the factorisation.
+ * Chameleon's zposv (compute/zposv.c) submits pzpotrf and the two pztrsm
+ * to one sequence, and its runtime starts each TRSM/GEMM task of the
+ * forward solve as soon as the factor tiles it reads are final.  Here the
+ * forward step of tile row k (fwd_tile + fwd_update, with the panel's own
+ * 64 x 64 block inverses) is issued on the main stream where it would
+ * otherwise wait for a bulk update, as long as those updates are long
+ * enough to hide it; the last tile rows, where the factorisation is a
+ * chain of short launches, are solved forward by one strip solve from
+ * their first strip, then the backward strip solve runs.  Used with the
+ * fused panel, lookahead and tiles a multiple of 64; otherwise dposv is
+ * dpotrf followed by dpotrs. */
+struct FwdSolve {
+    double *A, *dinv, *BT;
+    size_t dper;
+    int NT, ldt, nrp, nrhs, nb;
+    int next;                   /* first tile row not yet issued      */
+    int last;                   /* last tile row issued beside potrf  */
+    int *d_info;
+};
+
+static void fwd_after_panels(void *arg, int k)
+{
+    FwdSolve *f = (FwdSolve *)arg;
+    hipStream_t sm = ctx()->s_main;
+    for (; f->next <= k && f->next <= f->last; ++f->next) {
+        const int t = f->next;
+        const double nb = f->nb, below = (double)(f->NT - t - 1) * nb;
+        timed(CHAMAMD_KERNEL_SOLVE, (nb * nb + 2.0 * below * nb) * f->nrhs, sm, [&] {
+            k::fwd_tile(f->A, f->dinv + (size_t)t * f->dper, f->BT, t, f->NT, f->ldt, f->nrp, f->d_info, sm);
+            k::fwd_update(f->A, f->BT, t, f->NT, f->ldt, f->nrp, f->d_info, sm);
+        });
+    }
+}
+
+/* The last tile row whose forward step hides behind the bulk updates: the
+ * bulk update next to tile row t has about S = NT - t trailing columns
+ * and takes 2 nb^3 S (S + 1) flops; at the FP64 update rate of MI355X
+ * (~60 TFLOP/s) it must last 3 ms or more.  The forward steps of a pair of
+ * tile rows take 0.5-1 ms of the main stream next to the bulk, and the
+ * pairs before the factorisation turns into a chain of short launches need
+ * that margin (measured at N = 32768, nb = 512: a tail of 25-30 tile rows
+ * is best, 17 gains nothing). */
+static int fwd_overlap_last(int NT, int nb)
+{
+    const double nb3 = (double)nb * nb * nb;
+    int t = NT - 1;
+    while (t >= 0) {
+        const double S = NT - t;
+        if (2.0 * nb3 * S * (S + 1.0) / 60e12 >= 3e-3) break;
+        --t;
+    }
+    return t;
+}
+
+int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    if (A->m == 0) return CHAMELEON_SUCCESS;
+    if (B->n == 0) return potrf_async(uplo, A, sequence);
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, sm);
+    mirror_acquire(B, MIRROR_NORMAL, false, sm);
+    Mirror *MA = mirror_of(A), *MB = mirror_of(B);
+    if (!(c->panel && c->lookahead > 0 && MA->ldt % 64 == 0 && MB->ldt == MA->ldt)) {
+        potrf_async(uplo, A, sequence);
+        /* the solve kernels return at once when the device info flag is
+         * set, so a failed factorisation costs no host round trip here */
+        return potrs_async(uplo, A, B, sequence);
+    }
+    SeqState *st = seq_state(sequence);
+    const int NT = MA->mt, ldt = MA->ldt, nrp = MB->nt * MB->wdt;
+    FwdSolve f;
+    f.dper = (size_t)(ldt / 64) * 64 * 64;
+    /* one workspace: the block inverses of the factorisation (where
+     * potrf_run puts them), then the transposed right-hand sides */
+    double *ws = (double *)workspace(((size_t)NT * f.dper + (size_t)nrp * NT * ldt) * sizeof(double));
+    f.A = (double *)MA->d;
+    f.dinv = ws;
+    f.BT = ws + (size_t)NT * f.dper;
+    f.NT = NT; f.ldt = ldt; f.nrp = nrp; f.nrhs = B->n; f.nb = A->nb;
+    f.next = 0;
+    f.last = c->fwd_tail >= 0 ? NT - 1 - c->fwd_tail : fwd_overlap_last(NT, A->nb);
+    f.d_info = st->d_info;
+    k::rhs_transpose((double *)MB->d, f.BT, NT, ldt, MB->wdt, MB->nt, 1, sm);
+
+    PotrfPlan p;
+    p.A = f.A;
+    p.NT = NT;
+    p.ldt = ldt;
+    p.nb = A->nb;
+    p.nvalid_last = A->m - (NT - 1) * A->mb;
+    p.d_info = st->d_info;
+    p.after_panels = fwd_after_panels;
+    p.after_arg = &f;
+    potrf_run(p);
+    st->pending_info = true;
+    mirror_mark_written(A);
+    A->sync = 1;
+
+    /* the rows not solved beside the factorisation, then backward */
+    const int S = NT * ldt / 64, chunks = (nrp + 63) / 64, s0 = f.next * (ldt / 64);
+    int epoch = 0;
+    int *flags = strip_flags(4 * S, 2 * chunks, &epoch);
+    const double rows_left = (double)A->m - (double)f.next * A->nb;
+    timed(CHAMAMD_KERNEL_SOLVE, (rows_left * rows_left + (double)A->m * A->m) * B->n, sm, [&] {
+        k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 0, flags, epoch,
+                        st->d_info, sm, s0);
+        k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 1, flags,
+                        epoch + chunks, st->d_info, sm);
+        /* a failed factorisation leaves B as it was */
+        k::rhs_transpose((double *)MB->d, f.BT, NT, ldt, MB->wdt, MB->nt, 0, sm, st->d_info);
+    });
+    mirror_mark_written(B);
+    B->sync = 1;
+    return CHAMELEON_SUCCESS;
+}
+
 } /* namespace chx */
 
 extern "C" {
+
+int CHAMAMD_Set_Forward_Tail(int tiles)
+{
+    if (!ctx()) {
+        error("CHAMAMD_Set_Forward_Tail", "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    if (tiles < -1) {
+        error("CHAMAMD_Set_Forward_Tail", "illegal tile count");
+        return CHAMELEON_ERR_ILLEGAL_VALUE;
+    }
+    ctx()->fwd_tail = tiles;
+    return CHAMELEON_SUCCESS;
+}
 
 int CHAMELEON_dpotrf_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, RUNTIME_sequence_t *sequence,
                                 RUNTIME_request_t *request)
@@ -389,10 +528,7 @@ int CHAMELEON_dposv_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B,
         error("CHAMELEON_dposv_Tile", "illegal value of uplo");
         return request_fail(sequence, request, -1);
     }
-    potrf_async(uplo, A, sequence);
-    /* the solve kernels return at once when the device info flag is set,
-     * so a failed factorisation costs no host round trip here */
-    rc = potrs_async(uplo, A, B, sequence);
+    rc = posv_async(uplo, A, B, sequence);
     return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
 }
 

@@ -57,6 +57,7 @@ struct Context {
     int ib = 48;
     int translation = ChamInPlace;
     int lookahead = 1;
+    int fwd_tail = -1;            /* dposv: tile rows solved forward after potrf (-1: chosen) */
     int autotuning = 1;
     int warnings = 1;
     int nworkers = 1;
@@ -146,6 +147,13 @@ struct PotrfPlan {
      * operands, demoted panel alternating between P[0] and P[1] */
     int lp_mode = -1;
     void *P[4] = {nullptr, nullptr, nullptr, nullptr};   /* panel k in P[k % 2], or P[k % 4] with paired updates */
+    /* called where the main stream is about to wait for a bulk update
+     * (with lookahead it usually has time to spare there), when panels
+     * 0..k are issued on it: work the callback issues on the main stream
+     * runs once they are final (their tiles, and the 64 x 64 inverses of
+     * their diagonal blocks in the potrf workspace) */
+    void (*after_panels)(void *arg, int k) = nullptr;
+    void *after_arg = nullptr;
 };
 void potrf_run(const PotrfPlan &p);
 struct SolvePlan {
@@ -191,6 +199,11 @@ void trsm_right_lower_trans(double *panel, int ntiles, size_t tile_stride, int l
 void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, int ldt, const double *L0,
                                     long long lstride, const double *dinv0, long long dstride, const int *info,
                                     hipStream_t s);
+/* Trailing updates cover the tile pairs n <= m < NT of the columns n0 .. n1-1
+ * (n1 clamped to NT).  update_pairs returns how many pairs that is and sets
+ * *ncol to the launch mode: the column count, or 0 for the whole triangle
+ * n0 .. NT-1 (n1 >= NT and more than one column). */
+int update_pairs(int n0, int n1, int NT, int *ncol);
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
                      int kp = 1);
 void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
@@ -220,10 +233,16 @@ void cond_gemv(const double *A, int NT, int ldt, int N, const double *v, double 
 void cond_rank2(double *A, int NT, int ldt, int N, const double *v, const double *z, hipStream_t s);
 /* solve.hip */
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
-                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s);
+                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s, int s0 = 0);
 void solve_coop(const double *A, const double *TN, const double *TT, double *BT, double *W, int NT, int ldt, int nrhs,
                 int trans, const int *info, hipStream_t s);
-void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s);
+/* with info != null, nothing is copied once *info is nonzero */
+void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s,
+                   const int *info = nullptr);
+/* forward substitution of one tile row right after its panel (dposv) */
+void fwd_tile(const double *A, const double *D, double *BT, int k, int NT, int ldt, int nrp, const int *info,
+              hipStream_t s);
+void fwd_update(const double *A, double *BT, int k, int NT, int ldt, int nrp, const int *info, hipStream_t s);
 void tile_transpose(const double *src, double *dst, int ntiles, int ldt, hipStream_t s);
 void gemm_batched(int transA, int M, int N, int K, double alpha, const double *A, int lda, long long strideA,
                   const double *B, int ldb, long long strideB, double beta, double *C, int ldc, long long strideC,

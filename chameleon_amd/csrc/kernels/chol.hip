@@ -57,19 +57,24 @@ __device__ __forceinline__ void lstore_slice(double *lds, const v2d (&r)[L], int
 /* Trailing update of step k: A(m,n) -= A(m,k) A(n,k)^T for the tile pairs
  * (with kp = 2, the two panels k and k+1 at once: A(m,n) -= A(m,k) A(n,k)^T
  * + A(m,k+1) A(n,k+1)^T, one read and write of A(m,n) for both steps)
- * n <= m of either one column (n = n0, colmode) or the whole trailing
- * triangle n0 <= n <= m < NT.  Diagonal tiles (m == n) only write their
- * lower triangle: that is the SYRK of pzpotrf.c:112-117, the off-diagonal
- * pairs are its GEMMs (pzpotrf.c:123-129), all in one launch.
+ * n <= m of either ncol columns n0 .. n0+ncol-1 (ncol > 0, the lookahead
+ * columns) or the whole trailing triangle n0 <= n <= m < NT (ncol = 0).
+ * Diagonal tiles (m == n) only write their lower triangle: that is the SYRK
+ * of pzpotrf.c:112-117, the off-diagonal pairs are its GEMMs
+ * (pzpotrf.c:123-129), all in one launch.
  *
  * Block tile BM x BN, K staged through LDS in BK-deep slices with the next
  * slice prefetched into registers while the current one is multiplied.
- * Each wavefront owns a WM x WN sub-tile of (WM/16)x(WN/16) MFMA blocks.
- * Operands are swapped (D = B A^T) so each lane's results are consecutive
- * rows of one column: stores of 16 lanes form one 128-byte segment.      */
+ * Each wavefront owns a WM x WN sub-tile of (WM/16)x(WN/16) MFMA blocks;
+ * its operand fragments are read from LDS one 4-deep step ahead of the
+ * MFMAs that use them (two register sets), and the first step of the next
+ * slice right after the slice barrier, so LDS latency stays off the MFMA
+ * pipe.  Operands are swapped (D = B A^T) so each lane's results are
+ * consecutive rows of one column: stores of 16 lanes form one 128-byte
+ * segment.                                                                */
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update_kernel(
-    double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int colmode, int total_blocks, const int *info)
+    double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int ncol, int total_blocks, const int *info)
 {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NTH = NW * 64;
@@ -77,6 +82,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int LA = (BK * BM) / (2 * NTH);   /* v2d loads per thread for A */
     constexpr int LB = (BK * BN) / (2 * NTH);
+    constexpr int KS = BK / 4;
     static_assert(LA >= 1 && LB >= 1, "tile too small for the thread count");
     /* two LDS stages: stage s+1 is stored while stage s is multiplied,
      * one barrier per K slice */
@@ -90,9 +96,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
     const int bpt = tbm * tbn;
     const int pair = bid / bpt, b = bid % bpt;
     int m, n;
-    if (colmode) {
+    if (ncol > 0) {
+        /* column n holds the NT - n tiles n .. NT-1 */
+        int q = pair, len = NT - n0;
         n = n0;
-        m = n0 + pair;
+        while (q >= len && n + 1 < n0 + ncol) {
+            q -= len;
+            ++n;
+            --len;
+        }
+        m = n + q;
     } else {
         /* pair -> (i, j), i >= j, row-major over the lower triangle */
         int i = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
@@ -128,11 +141,21 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
         gload_slice<BM, LA, NTH>(ra, Am + pi * pstride, kb, ldt, tid);
         gload_slice<BN, LB, NTH>(rb, An + pi * pstride, kb, ldt, tid);
     };
+    /* operand fragments of one 4-deep step, two sets */
+    double fa[2][FM], fb[2][FN];
+    auto fload = [&](int buf, int kk, int set) {
+        const double *as = As[buf], *bs = Bs[buf];
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm) fa[set][fm] = as[(kk + (lane >> 4)) * BM + wm + fm * 16 + (lane & 15)];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) fb[set][fn] = bs[(kk + (lane >> 4)) * BN + wn + fn * 16 + (lane & 15)];
+    };
     gload(0);
     lstore_slice<BM, LA, NTH>(As[0], ra, tid);
     lstore_slice<BN, LB, NTH>(Bs[0], rb, tid);
     if (S > 1) gload(1);
     __syncthreads();
+    fload(0, 0, 0);
     for (int st = 0; st < S; ++st) {
         const int cur = st & 1;
         if (st + 1 < S) {
@@ -140,20 +163,22 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
             lstore_slice<BN, LB, NTH>(Bs[cur ^ 1], rb, tid);
             if (st + 2 < S) gload(st + 2);
         }
-        const double *as = As[cur], *bs = Bs[cur];
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double a[FM], bv[FN];
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm) a[fm] = as[(kk + (lane >> 4)) * BM + wm + fm * 16 + (lane & 15)];
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn) bv[fn] = bs[(kk + (lane >> 4)) * BN + wn + fn * 16 + (lane & 15)];
+        for (int q = 0; q < KS; ++q) {
+            const int set = q & 1;
+            if (q + 1 < KS) {
+                fload(cur, 4 * (q + 1), set ^ 1);
+            } else {
+                /* slice st+1 is in LDS and every wave is done reading
+                 * slice st (its buffer is refilled next iteration) */
+                __syncthreads();
+                if (st + 1 < S) fload(cur ^ 1, 0, set ^ 1);
+            }
 #pragma unroll
             for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = mfma_f64(bv[fn], a[fm], acc[fm][fn]);
+                for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = mfma_f64(fb[set][fn], fa[set][fm], acc[fm][fn]);
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int fm = 0; fm < FM; ++fm)
@@ -168,7 +193,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
 }
 
 template <int BM, int BN, int WM, int WN>
-static void launch_update(double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int colmode, int npairs,
+static void launch_update(double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int ncol, int npairs,
                           const int *info, hipStream_t s)
 {
     const int bpt = (ldt / BM) * (ldt / BN);
@@ -176,24 +201,34 @@ static void launch_update(double *A, int lmt, int ldt, int k, int kp, int n0, in
     if (total <= 0) return;
     constexpr int NTH = (BM / WM) * (BN / WN) * 64;
     hipLaunchKernelGGL((trailing_update_kernel<BM, BN, WM, WN>), dim3((unsigned)total), dim3(NTH), 0, s, A, lmt,
-                       ldt, k, kp, n0, NT, colmode, (int)total, info);
+                       ldt, k, kp, n0, NT, ncol, (int)total, info);
+}
+
+int update_pairs(int n0, int n1, int NT, int *ncol)
+{
+    n1 = n1 < NT ? n1 : NT;
+    const int S = NT - n0;
+    /* columns n0 .. n1-1 only, unless that is the whole triangle */
+    const int nc = (n1 < NT || n1 == n0 + 1) ? n1 - n0 : 0;
+    *ncol = nc;
+    if (nc == 0) return S * (S + 1) / 2;
+    return nc * S - nc * (nc - 1) / 2;
 }
 
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
                      int kp)
 {
     if (n0 >= NT || n1 <= n0) return;
-    const bool colmode = (n1 == n0 + 1);
-    const int S = NT - n0;
-    const int npairs = colmode ? S : S * (S + 1) / 2;
+    int ncol = 0;
+    const int npairs = update_pairs(n0, n1, NT, &ncol);
     if (ldt % 128 == 0)
-        launch_update<128, 128, 64, 32>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
+        launch_update<128, 128, 32, 64>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
     else if (ldt % 64 == 0)
-        launch_update<64, 64, 32, 32>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
+        launch_update<64, 64, 32, 32>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
     else if (ldt % 32 == 0)
-        launch_update<32, 32, 16, 16>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
+        launch_update<32, 32, 16, 16>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
     else
-        launch_update<16, 16, 16, 16>(A, lmt, ldt, k, kp, n0, NT, colmode, npairs, info, s);
+        launch_update<16, 16, 16, 16>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
 }
 
 /* ================================================================== */

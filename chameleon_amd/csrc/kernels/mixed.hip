@@ -85,12 +85,19 @@ void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void 
 }
 
 /* Tile-pair decoding shared by the update kernels (same enumeration as the
- * FP64 trailing_update_kernel: one column n0, or the triangle n0..NT-1). */
-__device__ __forceinline__ void decode_pair(int pair, int n0, int colmode, int &m, int &n)
+ * FP64 trailing_update_kernel: ncol columns from n0, or the triangle
+ * n0..NT-1 when ncol is 0). */
+__device__ __forceinline__ void decode_pair(int pair, int n0, int ncol, int NT, int &m, int &n)
 {
-    if (colmode) {
+    if (ncol > 0) {
+        int q = pair, len = NT - n0;
         n = n0;
-        m = n0 + pair;
+        while (q >= len && n + 1 < n0 + ncol) {
+            q -= len;
+            ++n;
+            --len;
+        }
+        m = n + q;
         return;
     }
     int i = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
@@ -105,7 +112,7 @@ __device__ __forceinline__ void decode_pair(int pair, int n0, int colmode, int &
  * wavefronts of 64 x 32, K in 32-deep LDS slices (double-buffered). */
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kernel(
-    double *A, const float *P, const float *P2, int lmt, int ldt, int n0, int colmode, int total_blocks,
+    double *A, const float *P, const float *P2, int lmt, int ldt, int n0, int ncol, int total_blocks,
     const int *info)
 {
     constexpr int NW = (BM / WM) * (BN / WN), NTH = NW * 64, BK = 32;
@@ -119,7 +126,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
     const int bid = xcd_remap_m(blockIdx.x, total_blocks);
     const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
     int m, n;
-    decode_pair(bid / bpt, n0, colmode, m, n);
+    decode_pair(bid / bpt, n0, ncol, lmt, m, n);   /* the tile grid is lmt x lmt, lmt = NT */
     const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
     const bool diag = (m == n);
     if (diag && col0 >= row0 + BM) return;
@@ -204,7 +211,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
 template <int SPLIT>
 __global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const unsigned short *P,
                                                              const unsigned short *P2, int NT, int lmt, int ldt,
-                                                             int n0, int colmode, int total_blocks, const int *info)
+                                                             int n0, int ncol, int total_blocks, const int *info)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 64, LDK = BK + 8;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const un
     const int bid = xcd_remap_m(blockIdx.x, total_blocks);
     const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
     int m, n;
-    decode_pair(bid / bpt, n0, colmode, m, n);
+    decode_pair(bid / bpt, n0, ncol, NT, m, n);
     const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
     const bool diag = (m == n);
     if (diag && col0 >= row0 + BM) return;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(512, 2) void update_bf16_kernel(double *A, const un
 template <int SPLIT>
 __global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const unsigned short *P,
                                                               const unsigned short *P2, int NT, int lmt, int ldt,
-                                                              int n0, int colmode, int total_blocks, const int *info)
+                                                              int n0, int ncol, int total_blocks, const int *info)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 32;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -315,7 +322,7 @@ __global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const u
     const int bid = xcd_remap_m(blockIdx.x, total_blocks);
     const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
     int m, n;
-    decode_pair(bid / bpt, n0, colmode, m, n);
+    decode_pair(bid / bpt, n0, ncol, NT, m, n);
     const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
     const bool diag = (m == n);
     if (diag && col0 >= row0 + BM) return;
@@ -414,15 +421,14 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
                         const int *info, hipStream_t s)
 {
     if (n0 >= NT || n1 <= n0) return;
-    const bool colmode = (n1 == n0 + 1);
-    const int S = NT - n0;
-    const int npairs = colmode ? S : S * (S + 1) / 2;
+    int ncol = 0;
+    const int npairs = update_pairs(n0, n1, NT, &ncol);
     const int bpt = (ldt / 128) * (ldt / 128);
     const long long total = (long long)npairs * bpt;
     if (total <= 0) return;
     if (mode == 0)
         hipLaunchKernelGGL((update_f32_kernel<128, 128, 64, 32>), dim3((unsigned)total), dim3(512), 0, s, A,
-                           (const float *)P, (const float *)P2, lmt, ldt, n0, (int)colmode, (int)total, info);
+                           (const float *)P, (const float *)P2, lmt, ldt, n0, ncol, (int)total, info);
     else {
         static const int pipelined = [] {
             const char *e = getenv("CHAMAMD_LP_PIPE");
@@ -431,7 +437,7 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
         auto *kern = pipelined ? (mode == 1 ? update_bf16p_kernel<1> : update_bf16p_kernel<3>)
                                : (mode == 1 ? update_bf16_kernel<1> : update_bf16_kernel<3>);
         hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
-                           (const unsigned short *)P2, NT, lmt, ldt, n0, (int)colmode, (int)total, info);
+                           (const unsigned short *)P2, NT, lmt, ldt, n0, ncol, (int)total, info);
     }
 }
 

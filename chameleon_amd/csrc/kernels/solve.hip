@@ -201,6 +201,8 @@ struct StripArgs {
     int NT, ldt, nrp, c0; /* c0: first column of this launch's chunks                 */
     int nchunk;           /* column chunks of 16*NC columns solved side by side       */
     int S;                /* number of strips                                         */
+    int s0;               /* forward: first strip solved (the rows above are solved and
+                           * already subtracted from the right-hand sides)           */
     int trans, epoch;
     const int *info;
 };
@@ -243,9 +245,9 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
     /* task t = (strip in chain order) * nchunk + chunk: the chunks are
      * independent chains, interleaved so that the tasks a workgroup waits
      * for always have smaller indices */
-    for (int t = blockIdx.x; t < a.S * a.nchunk; t += gridDim.x) {
+    for (int t = blockIdx.x; t < (a.S - a.s0) * a.nchunk; t += gridDim.x) {
         const int si = t / a.nchunk, cc = t - si * a.nchunk;
-        const int s = fwd ? si : a.S - 1 - si;
+        const int s = fwd ? a.s0 + si : a.S - 1 - si;
         const int c0 = a.c0 + cc * NCOL;
         int *const flags = a.flags + (size_t)cc * a.S;
         const int tm = s / q, rs = (s % q) * SR;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
         for (int e = tid; e < SR * SR; e += 256) Ds[(e % SR) + (e / SR) * (SR + 1)] = Dg[(e % SR) + (size_t)(e / SR) * a.ldd];
         const int pc0 = fwd ? 0 : wave, pc1 = fwd ? wave + 1 : NPC;   /* nonzero chunks of row group `wave` */
 
-        int ready = fwd ? -1 : a.S;
+        int ready = fwd ? a.s0 - 1 : a.S;
         auto wait_for = [&](int j) {
             if (fwd ? j <= ready : j >= ready) return;
             /* the lanes poll the flags of the next 64 strips; published
@@ -313,11 +315,11 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
          * Block j is multiplied before the wait for block j+1, so a strip
          * at the front of the chain has its partial sum ready when its
          * predecessor publishes. */
-        const int nblk = fwd ? s : a.S - 1 - s;
+        const int nblk = fwd ? s - a.s0 : a.S - 1 - s;
         if (nblk > 0) {
             double lcur[NPC][4], lnext[NPC][4];
             v2d xv[XV];
-            int j = fwd ? 0 : a.S - 1;
+            int j = fwd ? a.s0 : a.S - 1;
             {
                 const double *L = block_of(j);
 #pragma unroll
@@ -403,20 +405,22 @@ static void strip_launch(const StripArgs &a, hipStream_t s)
         if (per_cu < 1) per_cu = 1;
     }
     int grid = ncu * per_cu;
-    if (grid > a.S * a.nchunk) grid = a.S * a.nchunk;
+    if (grid > (a.S - a.s0) * a.nchunk) grid = (a.S - a.s0) * a.nchunk;
     StripArgs x = a;
     void *args[] = {&x};
     CHX_HIP(hipLaunchCooperativeKernel((const void *)strip_solve_kernel<NC>, dim3(grid), dim3(256), args, 0, s));
 }
 
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
-                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s)
+                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s, int s0)
 {
     StripArgs a;
     a.A = A; a.D = D; a.dts = dts; a.doff = doff; a.ldd = ldd;
     a.BT = BT; a.flags = flags;
     a.NT = NT; a.ldt = ldt; a.nrp = nrp;
     a.S = NT * ldt / SR;
+    a.s0 = trans ? 0 : s0;
+    if (a.s0 >= a.S) return 0;
     a.trans = trans;
     a.info = info;
     /* groups of up to 64 columns per launch; a group is split into chunks
@@ -450,11 +454,148 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
 }
 
 /* ------------------------------------------------------------------ */
+/* Forward substitution of dposv beside the factorisation, in the
+ * right-looking order of Chameleon's forward pztrsm (compute/pztrsm.c:
+ * 134-160: TRSM of B(k) by L(k,k), then GEMM of every B(m), m > k, by
+ * L(m,k)).  Both run as soon as panel k is factored: L(k,k), the tiles
+ * below it and the 64 x 64 inverses of its diagonal blocks (D, written by
+ * the panel kernel, element (r, p) of block b at D[b*4096 + r + p*64], zero
+ * above the diagonal) are final then.  BT holds the right-hand sides
+ * transposed (element (c, r) at c + r*nrp), solved in place.
+ *
+ * fwd_tile_kernel: one workgroup per 16*NC columns walks the 64-row
+ * strips b of tile k in order: acc = B_b - sum_{j<b} L(b,j) X_j over the
+ * strips of this tile (the earlier tiles were subtracted by fwd_update),
+ * then X_b = D_b acc.  Each wave owns 16 rows of the strip; the MFMA
+ * lane layout is the one of strip_solve_kernel (k index p0 + 4 lk + q).   */
+template <int NC>
+__global__ __launch_bounds__(256) void fwd_tile_kernel(const double *A, const double *D, double *BT, int k, int NT,
+                                                       int ldt, int nrp, const int *info)
+{
+    constexpr int NCOL = 16 * NC;
+    __shared__ double Xs[64 * NCOL];
+    if (*info != 0) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4, wr = wave * 16;
+    const size_t ts = (size_t)ldt * ldt;
+    const double *L = A + ((size_t)k * NT + k) * ts;
+    double *B = BT + blockIdx.x * NCOL + (size_t)k * ldt * nrp;
+    for (int b = 0; b < ldt / 64; ++b) {
+        v4d acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[c][e] = B[(c * 16 + li) + (size_t)(64 * b + wr + lk + 4 * e) * nrp];
+        for (int j = 0; j < b; ++j) {
+#pragma unroll
+            for (int pc = 0; pc < 4; ++pc) {
+                double l[4];
+                load_op4(L, ldt, false, 64 * b + wr, 64 * j + pc * 16, li, lk, l);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const size_t row = (size_t)(64 * j + pc * 16 + 4 * lk + q) * nrp;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) acc[c] = mfma_f64(-l[q], B[(c * 16 + li) + row], acc[c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Xs[(wr + lk + 4 * e) * NCOL + c * 16 + li] = acc[c][e];
+        __syncthreads();
+        const double *Db = D + (size_t)b * 4096;
+        v4d x[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) x[c] = zero4d();
+        for (int pc = 0; pc <= wave; ++pc) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int pp = pc * 16 + 4 * lk + q;
+                const double dv = Db[(wr + li) + pp * 64];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv, Xs[pp * NCOL + c * 16 + li], x[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) B[(c * 16 + li) + (size_t)(64 * b + wr + lk + 4 * e) * nrp] = x[c][e];
+        /* X_b is read by the later strips of this tile; Xs is refilled */
+        __syncthreads();
+    }
+}
+
+/* fwd_update_kernel: B(rows of tiles m > k) -= L(m,k) X_k.  Workgroup
+ * (x, y): 64 rows from row (k+1)*ldt + 64x, columns 16*NC*y ..; each wave
+ * 16 rows, the whole K = ldt in registers-only MFMA steps (L(m,k) is read
+ * once, X_k from the cache). */
+template <int NC>
+__global__ __launch_bounds__(256) void fwd_update_kernel(const double *A, double *BT, int k, int NT, int ldt, int nrp,
+                                                         const int *info)
+{
+    constexpr int NCOL = 16 * NC;
+    if (*info != 0) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4, wr = wave * 16;
+    const int R0 = (k + 1) * ldt + blockIdx.x * 64;
+    const int m = R0 / ldt, rr = R0 - m * ldt;
+    const size_t ts = (size_t)ldt * ldt;
+    const double *L = A + ((size_t)k * NT + m) * ts;
+    const double *X = BT + blockIdx.y * NCOL + (size_t)k * ldt * nrp;
+    double *Bo = BT + blockIdx.y * NCOL + (size_t)R0 * nrp;
+    v4d acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[c][e] = Bo[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp];
+#pragma unroll 2
+    for (int pc = 0; pc < ldt / 16; ++pc) {
+        double l[4];
+        load_op4(L, ldt, false, rr + wr, pc * 16, li, lk, l);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const size_t row = (size_t)(pc * 16 + 4 * lk + q) * nrp;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[c] = mfma_f64(-l[q], X[(c * 16 + li) + row], acc[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Bo[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp] = acc[c][e];
+}
+
+void fwd_tile(const double *A, const double *D, double *BT, int k, int NT, int ldt, int nrp, const int *info,
+              hipStream_t s)
+{
+    /* 16-column chains: the most workgroups on this short latency chain */
+    hipLaunchKernelGGL(fwd_tile_kernel<1>, dim3(nrp / 16), dim3(256), 0, s, A, D, BT, k, NT, ldt, nrp, info);
+}
+
+void fwd_update(const double *A, double *BT, int k, int NT, int ldt, int nrp, const int *info, hipStream_t s)
+{
+    const int rows = (NT - k - 1) * ldt;
+    if (rows <= 0) return;
+    if (nrp % 64 == 0)
+        hipLaunchKernelGGL(fwd_update_kernel<4>, dim3(rows / 64, nrp / 64), dim3(256), 0, s, A, BT, k, NT, ldt, nrp,
+                           info);
+    else if (nrp % 32 == 0)
+        hipLaunchKernelGGL(fwd_update_kernel<2>, dim3(rows / 64, nrp / 32), dim3(256), 0, s, A, BT, k, NT, ldt, nrp,
+                           info);
+    else
+        hipLaunchKernelGGL(fwd_update_kernel<1>, dim3(rows / 64, nrp / 16), dim3(256), 0, s, A, BT, k, NT, ldt, nrp,
+                           info);
+}
+
+/* ------------------------------------------------------------------ */
 /* B mirror (NT x bnt tiles of ldt x wdt) <-> BT (nrhs_p x NT*ldt), with
  * nrhs_p = bnt * wdt. */
-__global__ void rhs_transpose_kernel(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt)
+__global__ void rhs_transpose_kernel(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt,
+                                     const int *info)
 {
     __shared__ double S[32][33];
+    if (info && *info != 0) return;   /* failed factorisation: B keeps its values */
     const int nrp = bnt * wdt;
     const int rows = NT * ldt;
     const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;   /* rows of B, columns (rhs) of B */
@@ -486,11 +627,12 @@ __global__ void rhs_transpose_kernel(double *Bm, double *BT, int NT, int ldt, in
     }
 }
 
-void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s)
+void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s,
+                   const int *info)
 {
     const int rows = NT * ldt, nrp = bnt * wdt;
     dim3 grid((rows + 31) / 32, (nrp + 31) / 32);
-    hipLaunchKernelGGL(rhs_transpose_kernel, grid, dim3(256), 0, s, Bm, BT, NT, ldt, wdt, bnt, to_bt);
+    hipLaunchKernelGGL(rhs_transpose_kernel, grid, dim3(256), 0, s, Bm, BT, NT, ldt, wdt, bnt, to_bt, info);
 }
 
 /* Per-tile transpose of a batch of square tiles (TT -> TN). */

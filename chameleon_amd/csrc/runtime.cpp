@@ -342,6 +342,7 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     c->translation = env_int("CHAMELEON_TRANSLATION_MODE", ChamInPlace);
     c->panel = env_int("CHAMAMD_PANEL", 1);
     c->pairs = env_int("CHAMAMD_PAIRS", 1);
+    c->fwd_tail = env_int("CHAMAMD_FWD_TAIL", -1);
     CHX_HIP(hipSetDevice(c->device));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_update, hipStreamNonBlocking));

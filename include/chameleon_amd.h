@@ -33,6 +33,16 @@ enum {
 int CHAMAMD_Kernel_Stats(int kernel_class, long long *launches, double *total_ms, double *flops);
 int CHAMAMD_Kernel_Stats_Reset(void);
 
+/* dposv runs the forward substitution of a tile row right after its panel,
+ * beside the rest of the factorisation (as Chameleon's runtime overlaps the
+ * pztrsm tasks of zposv with pzpotrf); the last `tiles` tile rows are solved
+ * forward after the factorisation instead, by one strip solve.  tiles = -1
+ * (the default, or the CHAMAMD_FWD_TAIL environment variable) picks the
+ * count from the tile count and size.  Returns CHAMELEON_SUCCESS,
+ * CHAMELEON_ERR_ILLEGAL_VALUE for tiles < -1, or
+ * CHAMELEON_ERR_NOT_INITIALIZED. */
+int CHAMAMD_Set_Forward_Tail(int tiles);
+
 /* Products used by the trailing updates of the mixed-precision solve (the
  * matrix itself stays in FP64 storage, panels are factored in FP64). */
 enum {

@@ -227,6 +227,9 @@ def test_non_spd_info_fused_panel(ch, nb, uplo):
         A[j, j] = -1.0
         B = np.ones((n, 3), order="F")
         assert ch.lib.CHAMELEON_dposv(uplo, n, 3, _p(A), n, _p(B), n) == j + 1
+        # the solve is not applied (LAPACK dposv calls dpotrs only when info
+        # is 0), also when the forward solve runs beside the factorisation
+        assert np.array_equal(B, np.ones((n, 3), order="F"))
     # the context is usable afterwards
     A = oracle.dplgsy(float(n), UL_, n, 51)
     assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n) == 0
@@ -339,6 +342,38 @@ def test_lookahead_schedules_agree(ch):
     A0 = oracle.dplgsy(float(n), UL_, n, 51)
     for F in out:
         assert oracle.check_factorization(A0, F, L_)[0]
+
+
+@pytest.mark.parametrize("n,nb,nrhs", [(500, 512, 7), (1700, 256, 64), (1345, 192, 130), (2000, 320, 1),
+                                       (1536, 128, 48)])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_posv_forward_beside_factorisation(ch, n, nb, nrhs, uplo):
+    """dposv solves tile rows forward right after their panels (beside the
+    rest of the factorisation, with the panel's 64 x 64 block inverses) and
+    the last `tail` tile rows by one strip solve after it; dpotrf then dpotrs
+    solves everything after the factorisation.  Same factor, solutions equal
+    to rounding, for every split (tail 0: every tile row the schedule
+    reaches; -1: the default choice; a single tile has no rows below)."""
+    ch.set_param(ch.TILE_SIZE, nb)
+    A0 = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+    B0 = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    A2, B2 = A0.copy(order="F"), B0.copy(order="F")
+    assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A2), n) == 0
+    assert ch.lib.CHAMELEON_dpotrs(uplo, n, nrhs, _p(A2), n, _p(B2), n) == 0
+    tri = np.tril if uplo == L_ else np.triu
+    Afull = oracle.dplgsy(float(n), UL_, n, 51)
+    try:
+        for tail in (0, 2, -1):
+            say("posv forward beside potrf", n, nb, nrhs, uplo, tail)
+            assert ch.lib.CHAMAMD_Set_Forward_Tail(tail) == 0
+            A, B = A0.copy(order="F"), B0.copy(order="F")
+            assert ch.lib.CHAMELEON_dposv(uplo, n, nrhs, _p(A), n, _p(B), n) == 0
+            assert np.array_equal(tri(A), tri(A2))
+            assert oracle.rel_frobenius(B, B2) <= 1e-13
+            assert oracle.check_solve(Afull, B, oracle.dplrnt(n, nrhs, 5673))[0]
+    finally:
+        ch.lib.CHAMAMD_Set_Forward_Tail(-1)
+    assert ch.lib.CHAMAMD_Set_Forward_Tail(-2) == ch.ERR_ILLEGAL_VALUE
 
 
 def test_upper_is_transpose_of_lower(ch):
