@@ -204,6 +204,7 @@ struct StripArgs {
     int s0;               /* forward: first strip solved (the rows above are solved and
                            * already subtracted from the right-hand sides)           */
     int trans, epoch;
+    int pub;              /* 1: publish by coherent stores + completion, 0: release fence */
     const int *info;
 };
 
@@ -379,16 +380,34 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
                 for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv, Xs[p * NCOL + c * 16 + li], x[c]);
             }
         }
+        if (a.pub) {
+            /* publication by completion: X_s is written with device-scope
+             * (coherent, sc1) stores, which reach the memory side past this
+             * XCD's L2 by themselves; once every wave has waited for its
+             * stores to complete (vmcnt 0) and the workgroup has met at the
+             * barrier, the flag store follows them.  A release fence would
+             * write back the whole L2 for the same guarantee. */
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp] = x[c][e];
-        /* publication: the barrier completes every wave's stores, then one
-         * device-scope release by one thread covers them all */
-        __syncthreads();
-        if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int e = 0; e < 4; ++e)
+                    __hip_atomic_store(&Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp], x[c][e], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0x0F70);   /* vmcnt(0), expcnt and lgkmcnt not waited for */
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(&flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp] = x[c][e];
+            /* publication: the barrier completes every wave's stores, then one
+             * device-scope release by one thread covers them all */
+            __syncthreads();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(&flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
 }
@@ -423,6 +442,11 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
     if (a.s0 >= a.S) return 0;
     a.trans = trans;
     a.info = info;
+    static const int pub = [] {
+        const char *e = getenv("CHAMAMD_SOLVE_PUB");
+        return e ? atoi(e) : 1;
+    }();
+    a.pub = pub;
     /* groups of up to 64 columns per launch; a group is split into chunks
      * of 16 * SOLVE_NC columns that run as separate chains side by side
      * (shorter work per chain link; the factor rows are read once per
