@@ -476,27 +476,31 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
  * (element (j, p) at j + p*nrp).
  *
  * The row range of A is cut in RES_KS pieces: wavefront (rb, jc, ks)
- * forms the 16 x 64 block (rows 16 rb.., columns 64 jc..) of A(:, piece
- * ks) X(piece ks, :) into its own partial buffer, and a second kernel adds
- * the pieces in a fixed order (deterministic).  Blocks below the diagonal
- * tile are read as stored, blocks above as the transpose of the stored
- * lower tile (32-byte row vectors), the diagonal tile by triangle.  The
- * k loop runs in 16-deep chunks with the next chunk's loads in flight. */
+ * forms the (16 RG) x 64 block (rows 16 RG rb.., columns 64 jc..) of
+ * A(:, piece ks) X(piece ks, :) into its own partial buffer, and a second
+ * kernel adds the pieces in a fixed order (deterministic).  With RG = 2 a
+ * wave's two 16-row groups share every X operand it loads (0.75 instead of
+ * 1.25 vector loads per MFMA; ldt a multiple of 32 keeps both groups in
+ * one tile row).  Blocks below the diagonal tile are read as stored,
+ * blocks above as the transpose of the stored lower tile (32-byte row
+ * vectors), the diagonal tile by triangle.  The k loop runs in 16-deep
+ * chunks with the next chunk's loads in flight. */
 #define RES_KS 8
+template <int RG>
 __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int NT, int ldt, const double *XT, int nrp,
                                                         double *part)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int rows = NT * ldt;
     const int jchunks = (nrp + 63) / 64;
-    const int nrb = rows / 16;
-    /* the four waves of a workgroup take four consecutive 16-row blocks of
+    const int nrb = rows / (16 * RG);
+    /* the four waves of a workgroup take four consecutive row blocks of
      * one k range, so they read the same X chunks at the same time */
     const int groups = (nrb + 3) / 4;
     const int ks = blockIdx.x % RES_KS, rest = blockIdx.x / RES_KS;
     const int rb = (rest % groups) * 4 + (threadIdx.x >> 6), jc = rest / groups;
     if (jc >= jchunks || rb >= nrb) return;     /* no barriers below: waves are independent */
-    const int r0 = rb * 16;
+    const int r0 = rb * 16 * RG;
     const int m = r0 / ldt, i0 = r0 % ldt;
     const int j0 = jc * 64;
     const int nf = min(4, (nrp - j0) / 16);
@@ -504,7 +508,11 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
     /* this wave's k range: [kb, ke) in units of 16 */
     const int nk16 = rows / 16;
     const int kb = (int)((long long)nk16 * ks / RES_KS) * 16, ke = (int)((long long)nk16 * (ks + 1) / RES_KS) * 16;
-    v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+    v4d acc[RG][4];
+#pragma unroll
+    for (int g = 0; g < RG; ++g)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) acc[g][f] = zero4d();
     /* x operands of chunk p0: X(p0 + 4 lk + q, j0 + 16 f + li) */
     auto load_x = [&](int p0, double (&x)[4][4]) {
 #pragma unroll
@@ -514,63 +522,76 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
             for (int f = 0; f < 4; ++f) x[q][f] = (f < nf) ? xr[16 * f] : 0.0;
         }
     };
-    auto mma = [&](const double (&a)[4], const double (&x)[4][4]) {
+    auto mma = [&](const double (&a)[RG][4], const double (&x)[4][4]) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int f = 0; f < 4; ++f)
-                if (f < nf) acc[f] = mfma_f64(a[q], x[q][f], acc[f]);
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+                    if (f < nf) acc[g][f] = mfma_f64(a[g][q], x[q][f], acc[g][f]);
     };
     /* the k range, tile by tile: three loop shapes, each software-pipelined */
     for (int nt = kb / ldt; nt * ldt < ke; ++nt) {
         const int pb = max(kb, nt * ldt) - nt * ldt, pe = min(ke, (nt + 1) * ldt) - nt * ldt;
         if (nt < m) {
-            /* stored tile (m, nt): element (i0 + li, p) at T[i0 + li + p*ldt] */
+            /* stored tile (m, nt): element (i0 + 16 g + li, p) at T[16 g + p*ldt] */
             const double *T = A + ((size_t)nt * NT + m) * ts + i0 + li;
-            double a[4], x[4][4], an[4], xn[4][4];
+            double a[RG][4], x[4][4], an[RG][4], xn[4][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) a[q] = T[(size_t)(pb + 4 * lk + q) * ldt];
+            for (int g = 0; g < RG; ++g)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[g][q] = T[16 * g + (size_t)(pb + 4 * lk + q) * ldt];
             load_x(nt * ldt + pb, x);
             for (int p = pb; p < pe; p += 16) {
                 const bool more = p + 16 < pe;
                 if (more) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) an[q] = T[(size_t)(p + 16 + 4 * lk + q) * ldt];
+                    for (int g = 0; g < RG; ++g)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) an[g][q] = T[16 * g + (size_t)(p + 16 + 4 * lk + q) * ldt];
                     load_x(nt * ldt + p + 16, xn);
                 }
                 mma(a, x);
                 if (more) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        a[q] = an[q];
+#pragma unroll
+                        for (int g = 0; g < RG; ++g) a[g][q] = an[g][q];
 #pragma unroll
                         for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
                     }
                 }
             }
         } else if (nt > m) {
-            /* transpose of stored tile (nt, m): element (i0 + li, p) at T[p + (i0 + li)*ldt] */
+            /* transpose of stored tile (nt, m): element (i0 + 16 g + li, p) at
+             * T[p + 16 g ldt] */
             const double *T = A + ((size_t)m * NT + nt) * ts + (size_t)(i0 + li) * ldt;
-            double a[4], x[4][4], an[4], xn[4][4];
-            {
-                const v4d v = *(const v4d *)(T + pb + 4 * lk);
+            double a[RG][4], x[4][4], an[RG][4], xn[4][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) a[q] = v[q];
+            for (int g = 0; g < RG; ++g) {
+                const v4d v = *(const v4d *)(T + (size_t)16 * g * ldt + pb + 4 * lk);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[g][q] = v[q];
             }
             load_x(nt * ldt + pb, x);
             for (int p = pb; p < pe; p += 16) {
                 const bool more = p + 16 < pe;
                 if (more) {
-                    const v4d v = *(const v4d *)(T + p + 16 + 4 * lk);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) an[q] = v[q];
+                    for (int g = 0; g < RG; ++g) {
+                        const v4d v = *(const v4d *)(T + (size_t)16 * g * ldt + p + 16 + 4 * lk);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) an[g][q] = v[q];
+                    }
                     load_x(nt * ldt + p + 16, xn);
                 }
                 mma(a, x);
                 if (more) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        a[q] = an[q];
+#pragma unroll
+                        for (int g = 0; g < RG; ++g) a[g][q] = an[g][q];
 #pragma unroll
                         for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
                     }
@@ -579,27 +600,33 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
         } else {
             /* diagonal tile, lower triangle stored */
             const double *T = A + ((size_t)m * NT + m) * ts;
-            const int ii = i0 + li;
             for (int p = pb; p < pe; p += 16) {
-                double a[4], x[4][4];
+                double a[RG][4], x[4][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int pp = p + 4 * lk + q;
-                    a[q] = T[(ii >= pp) ? ii + (size_t)pp * ldt : pp + (size_t)ii * ldt];
+                for (int g = 0; g < RG; ++g) {
+                    const int ii = i0 + 16 * g + li;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int pp = p + 4 * lk + q;
+                        a[g][q] = T[(ii >= pp) ? ii + (size_t)pp * ldt : pp + (size_t)ii * ldt];
+                    }
                 }
                 load_x(nt * ldt + p, x);
                 mma(a, x);
             }
         }
     }
-    /* slot s of acc[f]: row r0 + lk + 4s, column j0 + 16 f + li */
+    /* slot s of acc[g][f]: row r0 + 16 g + lk + 4s, column j0 + 16 f + li */
     double *P = part + (size_t)ks * rows * nrp;
 #pragma unroll
-    for (int f = 0; f < 4; ++f)
-        if (f < nf) {
+    for (int g = 0; g < RG; ++g)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) P[(j0 + 16 * f + li) + (size_t)(r0 + lk + 4 * s) * nrp] = acc[f][s];
-        }
+        for (int f = 0; f < 4; ++f)
+            if (f < nf) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    P[(j0 + 16 * f + li) + (size_t)(r0 + 16 * g + lk + 4 * s) * nrp] = acc[g][f][s];
+            }
 }
 
 __global__ void residual_sum_kernel(const double *BT, const double *part, double *RT, size_t n)
@@ -617,8 +644,13 @@ size_t residual_scratch(int NT, int ldt, int nrp) { return (size_t)RES_KS * NT *
 void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s)
 {
-    const long long blocks = (long long)((NT * ldt / 16 + 3) / 4) * ((nrp + 63) / 64) * RES_KS;
-    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
+    /* two 16-row groups per wave when both lie in one tile row */
+    const int rg = (ldt % 32 == 0) ? 2 : 1;
+    const long long blocks = (long long)((NT * ldt / (16 * rg) + 3) / 4) * ((nrp + 63) / 64) * RES_KS;
+    if (rg == 2)
+        hipLaunchKernelGGL(residual_kernel<2>, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
+    else
+        hipLaunchKernelGGL(residual_kernel<1>, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
     const size_t n = (size_t)NT * ldt * nrp;
     hipLaunchKernelGGL(residual_sum_kernel, dim3(2048), dim3(256), 0, s, BT, part, RT, n);
 }
