@@ -221,7 +221,22 @@ void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT,
     if (n0 >= NT || n1 <= n0) return;
     int ncol = 0;
     const int npairs = update_pairs(n0, n1, NT, &ncol);
-    if (ldt % 128 == 0)
+    /* A column launch too small to give every compute unit two 128 x 128
+     * blocks is latency-bound (each block walks the whole K alone): it
+     * takes 64 x 64 blocks, four times as many with a quarter of the work
+     * each (the same k order per element, so the same sums). */
+    static const int ncu = [] {
+        int dev = 0, n = 0;
+        CHX_HIP(hipGetDevice(&dev));
+        CHX_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        return n;
+    }();
+    static const bool use_small = [] {
+        const char *e = getenv("CHAMAMD_SMALL_COLS");
+        return !(e && atoi(e) == 0);
+    }();
+    const bool small_cols = use_small && ncol > 0 && (long long)npairs * (ldt / 128) * (ldt / 128) < 2LL * ncu;
+    if (ldt % 128 == 0 && !small_cols)
         launch_update<128, 128, 32, 64>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
     else if (ldt % 64 == 0)
         launch_update<64, 64, 32, 32>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
