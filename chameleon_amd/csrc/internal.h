@@ -227,6 +227,13 @@ void residual(const double *A, int NT, int ldt, const double *XT, const double *
 size_t residual_scratch(int NT, int ldt, int nrp);   /* doubles of `part` */
 void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStream_t s);
 void axpy(double *Y, const double *X, size_t n, hipStream_t s);
+/* lower tiles of S copied to W and the symmetric inf-norm's row sums into
+ * rowsum[N]; returns -1 (nothing launched) unless ldt is a multiple of 128
+ * up to COPY_NORM_MAX_LDT and the tiling of N by mb fills NT tile rows */
+#define COPY_NORM_MAX_LDT 2048
+int copy_lower_norm(const double *S, double *W, int NT, int ldt, int mb, int N, double *part, double *rowsum,
+                    hipStream_t s);
+size_t copy_lower_norm_scratch(int NT, int ldt);     /* doubles of `part` */
 /* cond.hip */
 void cond_init(double *A, int NT, int ldt, int N, double kappa, hipStream_t s);
 void cond_gemv(const double *A, int NT, int ldt, int N, const double *v, double *w, hipStream_t s);

@@ -684,5 +684,92 @@ void axpy(double *Y, const double *X, size_t n, hipStream_t s)
     hipLaunchKernelGGL(axpy_kernel, dim3(2048), dim3(256), 0, s, Y, X, n);
 }
 
+/* ------------------------------------------------------------------ */
+/* Working copy of the lower tiles of a symmetric tile matrix, with the
+ * row sums of its inf-norm formed in the same pass (one read of A instead
+ * of a copy and a norm pass).  S and W hold NT x NT tiles of ldt x ldt,
+ * tile (m, n) at (n NT + m) ldt^2; only tiles m >= n are read and written.
+ * The valid part of tile (m, n) is min(mb, N - m mb) x min(mb, N - n mb);
+ * of a diagonal tile only the lower triangle counts.  For tile t = n NT + m:
+ *   part[2 t ldt + i]       = sum_j |S(i, j)|                  (rows)
+ *   part[(2 t + 1) ldt + j] = sum_i |S(i, j)|, diagonal excluded (columns)
+ * so logical row r = m mb + i of the symmetric matrix is the sum of the
+ * tile rows i of tiles (m, 0..m) and the tile columns i of (m..NT-1, m).
+ * One workgroup per tile: wave w takes rows w*128 + 512 c, two rows per
+ * lane (16-byte loads), eight columns in flight; each column's sum goes
+ * through a wave reduction into an LDS row of the wave's own (no atomics:
+ * the result does not depend on scheduling).  ldt a multiple of 128. */
+__global__ __launch_bounds__(256) void copy_lower_norm_kernel(const double *S, double *W, int NT, int ldt, int mb,
+                                                              int N, double *part)
+{
+    extern __shared__ double cs[];      /* [4][ldt] column sums, one row per wave */
+    const int m = blockIdx.x % NT, n = blockIdx.x / NT;
+    if (m < n) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t ts = (size_t)ldt * ldt, t = (size_t)n * NT + m;
+    const double *src = S + t * ts;
+    double *dst = W + t * ts;
+    const int vm = min(mb, N - m * mb), vn = min(mb, N - n * mb);
+    const bool diag = (m == n);
+    double *csw = cs + (size_t)wave * ldt;
+    for (int j = lane; j < ldt; j += 64) csw[j] = 0.0;
+    for (int r0 = wave * 128; r0 < ldt; r0 += 512) {
+        const int i = r0 + 2 * lane;
+        double rs0 = 0.0, rs1 = 0.0;
+        for (int j0 = 0; j0 < ldt; j0 += 8) {
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2 *>(src + (size_t)(j0 + u) * ldt + i);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + u;
+                *reinterpret_cast<double2 *>(dst + (size_t)j * ldt + i) = v[u];
+                const bool cj = j < vn;
+                const double a0 = (cj && i < vm && (!diag || i >= j)) ? fabs(v[u].x) : 0.0;
+                const double a1 = (cj && i + 1 < vm && (!diag || i + 1 >= j)) ? fabs(v[u].y) : 0.0;
+                rs0 += a0;
+                rs1 += a1;
+                double c = (diag && i == j ? 0.0 : a0) + (diag && i + 1 == j ? 0.0 : a1);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+                if (lane == 0) csw[j] += c;
+            }
+        }
+        part[2 * t * ldt + i] = rs0;
+        part[2 * t * ldt + i + 1] = rs1;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < ldt; j += 256)
+        part[(2 * t + 1) * ldt + j] = ((cs[j] + cs[ldt + j]) + cs[2 * ldt + j]) + cs[3 * ldt + j];
+}
+
+/* rowsum[r], r < N: the symmetric matrix's absolute row sums from the
+ * partials of copy_lower_norm_kernel, added in a fixed order */
+__global__ void norm_rows_kernel(const double *part, int NT, int ldt, int mb, int N, double *rowsum)
+{
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+        const int m = r / mb, i = r % mb;
+        double s = 0.0;
+        for (int n = 0; n <= m; ++n) s += part[2 * ((size_t)n * NT + m) * ldt + i];
+        for (int q = m; q < NT; ++q) s += part[(2 * ((size_t)m * NT + q) + 1) * ldt + i];
+        rowsum[r] = s;
+    }
+}
+
+size_t copy_lower_norm_scratch(int NT, int ldt) { return (size_t)2 * NT * NT * ldt; }
+
+int copy_lower_norm(const double *S, double *W, int NT, int ldt, int mb, int N, double *part, double *rowsum,
+                    hipStream_t s)
+{
+    if (NT <= 0 || ldt % 128 != 0 || ldt > COPY_NORM_MAX_LDT || mb <= 0 || mb > ldt || N <= (NT - 1) * mb ||
+        N > NT * mb)
+        return -1;
+    hipLaunchKernelGGL(copy_lower_norm_kernel, dim3((unsigned)(NT * NT)), dim3(256), (size_t)4 * ldt * sizeof(double),
+                       s, S, W, NT, ldt, mb, N, part);
+    hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)std::min(2048, (N + 255) / 256)), dim3(256), 0, s, part, NT,
+                       ldt, mb, N, rowsum);
+    return 0;
+}
+
 } /* namespace k */
 } /* namespace chx */

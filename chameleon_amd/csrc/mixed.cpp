@@ -23,6 +23,7 @@
 #include "chameleon_amd.h"
 #include "chameleon/chameleon_ds.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -150,17 +151,28 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
     SeqState *st = seq_state(seq);
 
     if (lp_ok) {
-        const double anrm = norm_inf_sym(A);
-        const double cte = anrm * eps * std::sqrt((double)N);
-        /* workspace2: working factor | panels (4) | BT | XT | RT | norms | residual pieces */
+        /* workspace2: working factor | panels (4) | BT | XT | RT | norms | residual pieces
+         * (the pieces hold the norm partials first) */
         const size_t abytes = MA->bytes;
         const size_t pbytes = (size_t)NT * ts * 4;
         const size_t vb = (size_t)nrp * rows;
-        const size_t pv = k::residual_scratch(NT, ldt, nrp);
+        const size_t pv = std::max(k::residual_scratch(NT, ldt, nrp), k::copy_lower_norm_scratch(NT, ldt));
         char *w = (char *)workspace2(abytes + 4 * pbytes + (3 * vb + nrp + pv) * sizeof(double));
         double *Aw = (double *)w;
         double *BT = (double *)(w + abytes + 4 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb, *part = nw + nrp;
-        CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
+        /* the factorisation reads and writes the lower tiles only: copy
+         * those and form ||A||_inf in the same pass (row sums into RT) */
+        double anrm = 0.0;
+        if (k::copy_lower_norm((const double *)MA->d, Aw, NT, ldt, A->mb, N, part, RT, s) == 0) {
+            std::vector<double> h(N);
+            CHX_HIP(hipMemcpyAsync(h.data(), RT, N * sizeof(double), hipMemcpyDeviceToHost, s));
+            CHX_HIP(hipStreamSynchronize(s));
+            for (double v : h) anrm = std::fmax(anrm, v);
+        } else {
+            anrm = norm_inf_sym(A);
+            CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
+        }
+        const double cte = anrm * eps * std::sqrt((double)N);
 
         PotrfPlan p;
         p.A = Aw;

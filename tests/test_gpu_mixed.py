@@ -149,6 +149,41 @@ def test_mixed_non_spd_reports_info(ch, golden):
     assert rc == info_ref > 0
 
 
+@pytest.mark.parametrize("uplo,n,nb,nrhs,prec", [(L_, 1150, 128, 3, BF16X3), (U_, 1150, 128, 3, FP32),
+                                                  (L_, 2048, 512, 8, BF16X3), (U_, 1000, 256, 2, BF16X3)])
+def test_mixed_norm_matches_backward_error(ch, uplo, n, nb, nrhs, prec):
+    """The refinement forms ||A||_inf in the same pass as its working copy
+    of A's lower tiles.  The backward error it reports for the returned X
+    must equal CHAMAMD_Backward_Error's, which recomputes the same residual
+    (same kernel, same operands) with the separate norm kernel: the ratio of
+    the two is the ratio of the two norms.  A dropped transposed half, a
+    counted padding row or a doubled diagonal moves it by percents here
+    (dense A, last tile ragged for 1150/128 and 1000/256)."""
+    say("mixed norm", uplo, n, nb, nrhs, prec)
+    A = conditioned_spd(n, 1e3, 11)
+    if uplo == L_:
+        A[np.triu_indices(n, 1)] = 7.0       # outside the stored triangle: must not count
+    else:
+        A[np.tril_indices(n, -1)] = 7.0
+    A0 = A.copy(order="F")
+    rng = np.random.default_rng(5)
+    B = np.asfortranarray(rng.uniform(-0.5, 0.5, (n, nrhs)))
+    rc, X, it, be = mixed(ch, uplo, A, B, prec, nb)
+    assert rc == 0 and it >= 0, it
+    assert np.array_equal(A, A0)
+    ref = ctypes.c_double(-1)
+    with ch.Desc.from_array(A0, nb) as dA, ch.Desc.from_array(X, nb) as dX, ch.Desc.from_array(B, nb) as dB:
+        assert ch.lib.CHAMAMD_Backward_Error(uplo, dA.ptr, dX.ptr, dB.ptr, ctypes.byref(ref)) == 0
+    assert 0.0 < be < 1.0
+    assert abs(be / ref.value - 1.0) <= 1e-12, (be, ref.value)
+    # and the norm itself: the backward error with numpy's ||A||_inf of the symmetric A
+    S = np.tril(A0) + np.tril(A0, -1).T if uplo == L_ else np.triu(A0) + np.triu(A0, 1).T
+    R = B - S @ X
+    ref_np = max(np.abs(R[:, j]).max() / (np.abs(S).sum(1).max() * np.abs(X[:, j]).max() * n * 2.0 ** -53)
+                 for j in range(nrhs))
+    assert 0.25 <= be / ref_np <= 4.0
+
+
 def test_backward_error_of_fp64_solution(ch):
     n, nb, nrhs = 2048, 256, 8
     say("backward error")
