@@ -395,22 +395,60 @@ static void fwd_after_panels(void *arg, int k)
 
 /* The last tile row whose forward step hides behind the bulk updates: the
  * bulk update next to tile row t has about S = NT - t trailing columns
- * and takes 2 nb^3 S (S + 1) flops; at the FP64 update rate of MI355X
- * (~60 TFLOP/s) it must last 3 ms or more.  The forward steps of a pair of
- * tile rows take 0.5-1 ms of the main stream next to the bulk, and the
+ * and takes 2 nb^3 S (S + 1) flops; at the update rate `rate` (flop/s; FP64
+ * on MI355X ~60e12) it must last 3 ms or more.  The forward steps of a pair
+ * of tile rows take 0.5-1 ms of the main stream next to the bulk, and the
  * pairs before the factorisation turns into a chain of short launches need
- * that margin (measured at N = 32768, nb = 512: a tail of 25-30 tile rows
- * is best, 17 gains nothing). */
-static int fwd_overlap_last(int NT, int nb)
+ * that margin (measured at N = 32768, nb = 512 in FP64: a tail of 25-30
+ * tile rows is best, 17 gains nothing). */
+static int fwd_overlap_last(int NT, int nb, double rate)
 {
     const double nb3 = (double)nb * nb * nb;
     int t = NT - 1;
     while (t >= 0) {
         const double S = NT - t;
-        if (2.0 * nb3 * S * (S + 1.0) / 60e12 >= 3e-3) break;
+        if (2.0 * nb3 * S * (S + 1.0) / rate >= 3e-3) break;
         --t;
     }
     return t;
+}
+
+bool potrf_solve_eligible(int ldt)
+{
+    Context *c = ctx();
+    return c->panel && c->lookahead > 0 && ldt % 64 == 0;
+}
+
+void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    const int NT = p.NT, ldt = p.ldt;
+    FwdSolve f;
+    f.dper = (size_t)(ldt / 64) * 64 * 64;
+    /* where potrf_run puts the block inverses */
+    f.dinv = (double *)workspace((size_t)NT * f.dper * sizeof(double));
+    f.A = p.A;
+    f.BT = BT;
+    f.NT = NT; f.ldt = ldt; f.nrp = nrp; f.nrhs = nrhs; f.nb = p.nb;
+    f.next = 0;
+    f.last = c->fwd_tail >= 0 ? NT - 1 - c->fwd_tail : fwd_overlap_last(NT, p.nb, rate);
+    f.d_info = p.d_info;
+    p.after_panels = fwd_after_panels;
+    p.after_arg = &f;
+    potrf_run(p);
+
+    /* the rows not solved beside the factorisation, then backward */
+    const int S = NT * ldt / 64, chunks = (nrp + 63) / 64, s0 = f.next * (ldt / 64);
+    int epoch = 0;
+    int *flags = strip_flags(4 * S, 2 * chunks, &epoch);
+    const double n = (double)(NT - 1) * p.nb + p.nvalid_last, rows_left = n - (double)f.next * p.nb;
+    timed(CHAMAMD_KERNEL_SOLVE, (rows_left * rows_left + n * n) * nrhs, sm, [&] {
+        k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 0, flags, epoch,
+                        p.d_info, sm, s0);
+        k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 1, flags,
+                        epoch + chunks, p.d_info, sm);
+    });
 }
 
 int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequence_t *sequence)
@@ -422,7 +460,7 @@ int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequenc
     mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, sm);
     mirror_acquire(B, MIRROR_NORMAL, false, sm);
     Mirror *MA = mirror_of(A), *MB = mirror_of(B);
-    if (!(c->panel && c->lookahead > 0 && MA->ldt % 64 == 0 && MB->ldt == MA->ldt)) {
+    if (!(potrf_solve_eligible(MA->ldt) && MB->ldt == MA->ldt)) {
         potrf_async(uplo, A, sequence);
         /* the solve kernels return at once when the device info flag is
          * set, so a failed factorisation costs no host round trip here */
@@ -430,47 +468,26 @@ int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequenc
     }
     SeqState *st = seq_state(sequence);
     const int NT = MA->mt, ldt = MA->ldt, nrp = MB->nt * MB->wdt;
-    FwdSolve f;
-    f.dper = (size_t)(ldt / 64) * 64 * 64;
+    const size_t dper = (size_t)(ldt / 64) * 64 * 64;
     /* one workspace: the block inverses of the factorisation (where
      * potrf_run puts them), then the transposed right-hand sides */
-    double *ws = (double *)workspace(((size_t)NT * f.dper + (size_t)nrp * NT * ldt) * sizeof(double));
-    f.A = (double *)MA->d;
-    f.dinv = ws;
-    f.BT = ws + (size_t)NT * f.dper;
-    f.NT = NT; f.ldt = ldt; f.nrp = nrp; f.nrhs = B->n; f.nb = A->nb;
-    f.next = 0;
-    f.last = c->fwd_tail >= 0 ? NT - 1 - c->fwd_tail : fwd_overlap_last(NT, A->nb);
-    f.d_info = st->d_info;
-    k::rhs_transpose((double *)MB->d, f.BT, NT, ldt, MB->wdt, MB->nt, 1, sm);
+    double *ws = (double *)workspace(((size_t)NT * dper + (size_t)nrp * NT * ldt) * sizeof(double));
+    double *BT = ws + (size_t)NT * dper;
+    k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, sm);
 
     PotrfPlan p;
-    p.A = f.A;
+    p.A = (double *)MA->d;
     p.NT = NT;
     p.ldt = ldt;
     p.nb = A->nb;
     p.nvalid_last = A->m - (NT - 1) * A->mb;
     p.d_info = st->d_info;
-    p.after_panels = fwd_after_panels;
-    p.after_arg = &f;
-    potrf_run(p);
+    potrf_solve_run(p, BT, nrp, B->n, 60e12);
     st->pending_info = true;
     mirror_mark_written(A);
     A->sync = 1;
-
-    /* the rows not solved beside the factorisation, then backward */
-    const int S = NT * ldt / 64, chunks = (nrp + 63) / 64, s0 = f.next * (ldt / 64);
-    int epoch = 0;
-    int *flags = strip_flags(4 * S, 2 * chunks, &epoch);
-    const double rows_left = (double)A->m - (double)f.next * A->nb;
-    timed(CHAMAMD_KERNEL_SOLVE, (rows_left * rows_left + (double)A->m * A->m) * B->n, sm, [&] {
-        k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 0, flags, epoch,
-                        st->d_info, sm, s0);
-        k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 1, flags,
-                        epoch + chunks, st->d_info, sm);
-        /* a failed factorisation leaves B as it was */
-        k::rhs_transpose((double *)MB->d, f.BT, NT, ldt, MB->wdt, MB->nt, 0, sm, st->d_info);
-    });
+    /* a failed factorisation leaves B as it was */
+    k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 0, sm, st->d_info);
     mirror_mark_written(B);
     B->sync = 1;
     return CHAMELEON_SUCCESS;

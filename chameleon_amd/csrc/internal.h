@@ -156,6 +156,14 @@ struct PotrfPlan {
     void *after_arg = nullptr;
 };
 void potrf_run(const PotrfPlan &p);
+/* potrf_run with the forward solve of BT (transposed right-hand sides,
+ * nrp x NT*ldt, nrhs of them in use) issued beside the factorisation, then
+ * the forward strips it left and the backward solve: BT <- A^-1 BT.  The
+ * solve kernels return at once when the device info flag is set.  `rate`:
+ * the bulk update rate (flop/s) the overlap is sized for.  Requires
+ * potrf_solve_eligible(ldt). */
+bool potrf_solve_eligible(int ldt);
+void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate);
 struct SolvePlan {
     double *TT, *TN, *BT, *W, *dinv;   /* diagonal-tile inverses, transposed RHS, one block, 64x64 inverses */
     int R = 0;                         /* 64: strip solve; 0: tile-step solve */

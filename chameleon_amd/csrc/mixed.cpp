@@ -174,6 +174,8 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         }
         const double cte = anrm * eps * std::sqrt((double)N);
 
+        k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
+        CHX_HIP(hipMemcpyAsync(XT, BT, vb * sizeof(double), hipMemcpyDeviceToDevice, s));
         PotrfPlan p;
         p.A = Aw;
         p.NT = NT;
@@ -183,16 +185,23 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         p.d_info = st->d_info;
         p.lp_mode = prec;
         for (int i = 0; i < 4; ++i) p.P[i] = w + abytes + i * pbytes;
-        potrf_run(p);
+        /* X0 = A_lp^-1 B: with the fused panel the forward solve runs beside
+         * the factorisation, sized for the rate of this precision's bulk
+         * update (measured on MI355X, FP64-equivalent flop/s) */
+        const bool overlap = potrf_solve_eligible(ldt);
+        if (overlap) {
+            static const double rate[3] = {98e12, 377e12, 205e12};     /* FP32, BF16, BF16x3 */
+            potrf_solve_run(p, XT, nrp, X->n, rate[prec]);
+        } else {
+            potrf_run(p);
+        }
         st->pending_info = true;
         const int info = sequence_wait_internal(seq);
         if (info != 0) {
             it_count = -3;
         } else {
             SolvePlan sp = solve_plan(Aw, NT, ldt, nrp, st->d_info);
-            k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
-            CHX_HIP(hipMemcpyAsync(XT, BT, vb * sizeof(double), hipMemcpyDeviceToDevice, s));
-            solve_bt(sp, Aw, NT, ldt, XT, nrp, st->d_info);
+            if (!overlap) solve_bt(sp, Aw, NT, ldt, XT, nrp, st->d_info);
             for (int it = 0; it <= ITERMAX; ++it) {
                 k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, part, s);
                 std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
