@@ -150,7 +150,8 @@ def test_mixed_non_spd_reports_info(ch, golden):
 
 
 @pytest.mark.parametrize("uplo,n,nb,nrhs,prec", [(L_, 1150, 128, 3, BF16X3), (U_, 1150, 128, 3, FP32),
-                                                  (L_, 2048, 512, 8, BF16X3), (U_, 1000, 256, 2, BF16X3)])
+                                                  (L_, 2048, 512, 8, BF16X3), (U_, 1000, 256, 2, BF16X3),
+                                                  (L_, 1000, 120, 2, BF16X3), (U_, 1000, 120, 2, FP32)])
 def test_mixed_norm_matches_backward_error(ch, uplo, n, nb, nrhs, prec):
     """The refinement forms ||A||_inf in the same pass as its working copy
     of A's lower tiles.  The backward error it reports for the returned X
@@ -158,7 +159,8 @@ def test_mixed_norm_matches_backward_error(ch, uplo, n, nb, nrhs, prec):
     (same kernel, same operands) with the separate norm kernel: the ratio of
     the two is the ratio of the two norms.  A dropped transposed half, a
     counted padding row or a doubled diagonal moves it by percents here
-    (dense A, last tile ragged for 1150/128 and 1000/256)."""
+    (dense A, last tile ragged for 1150/128 and 1000/256; nb 120 pads every
+    tile to 128 rows)."""
     say("mixed norm", uplo, n, nb, nrhs, prec)
     A = conditioned_spd(n, 1e3, 11)
     if uplo == L_:
