@@ -36,9 +36,11 @@ int CHAMAMD_Kernel_Stats_Reset(void);
 /* dposv runs the forward substitution of a tile row right after its panel,
  * beside the rest of the factorisation (as Chameleon's runtime overlaps the
  * pztrsm tasks of zposv with pzpotrf); the last `tiles` tile rows are solved
- * forward after the factorisation instead, by one strip solve.  tiles = -1
+ * forward after the factorisation instead, by one strip solve.  The first
+ * solve of the mixed-precision refinement runs the same way.  tiles = -1
  * (the default, or the CHAMAMD_FWD_TAIL environment variable) picks the
- * count from the tile count and size.  Returns CHAMELEON_SUCCESS,
+ * count from the tile count and size and, in the mixed-precision solve, the
+ * update rate of the chosen products.  Returns CHAMELEON_SUCCESS,
  * CHAMELEON_ERR_ILLEGAL_VALUE for tiles < -1, or
  * CHAMELEON_ERR_NOT_INITIALIZED. */
 int CHAMAMD_Set_Forward_Tail(int tiles);
