@@ -713,6 +713,7 @@ __global__ __launch_bounds__(256) void copy_lower_norm_kernel(const double *S, d
     const bool diag = (m == n);
     double *csw = cs + (size_t)wave * ldt;
     for (int j = lane; j < ldt; j += 64) csw[j] = 0.0;
+    __syncthreads();            /* lane 0 adds below into entries other lanes cleared */
     for (int r0 = wave * 128; r0 < ldt; r0 += 512) {
         const int i = r0 + 2 * lane;
         double rs0 = 0.0, rs1 = 0.0;
