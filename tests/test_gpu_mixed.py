@@ -186,6 +186,36 @@ def test_mixed_norm_matches_backward_error(ch, uplo, n, nb, nrhs, prec):
     assert 0.25 <= be / ref_np <= 4.0
 
 
+@pytest.mark.parametrize("n,nb,nrhs", [(1536, 128, 48), (2000, 256, 5), (1150, 128, 7)])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_mixed_forward_beside_factorisation(ch, n, nb, nrhs, uplo):
+    """The first solve of the refinement runs forward beside the
+    factorisation (tile rows up to the tail) and by strip solve after it.
+    A wrong first solve would still be corrected by refinement, so besides
+    the answer the step count must match the run that solves every row
+    after the factorisation (tail 1000 tile rows: none beside it)."""
+    A0 = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    Xr = np.linalg.solve(oracle.dplgsy(float(n), UL_, n, 51), B)
+    try:
+        ref = None
+        for tail in (1000, 0, 2, -1):
+            say("mixed forward beside potrf", n, nb, nrhs, uplo, tail)
+            assert ch.lib.CHAMAMD_Set_Forward_Tail(tail) == 0
+            A = A0.copy(order="F")
+            rc, X, it, be = mixed(ch, uplo, A, B, BF16X3, nb)
+            assert rc == 0 and it >= 0, it
+            assert np.array_equal(A, A0)
+            assert oracle.rel_frobenius(X, Xr) <= 1e-13
+            if ref is None:
+                ref = (it, X)
+            else:
+                assert it == ref[0], (tail, it, ref[0])
+                assert oracle.rel_frobenius(X, ref[1]) <= 1e-13
+    finally:
+        ch.lib.CHAMAMD_Set_Forward_Tail(-1)
+
+
 def test_backward_error_of_fp64_solution(ch):
     n, nb, nrhs = 2048, 256, 8
     say("backward error")
