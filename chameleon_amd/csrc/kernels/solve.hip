@@ -449,10 +449,10 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
     a.pub = pub;
     /* groups of up to 64 columns per launch; a group is split into chunks
      * of 16 * SOLVE_NC columns that run as separate chains side by side
-     * (shorter work per chain link).  Each chunk reads the factor rows
-     * from HBM itself: at N = 32768, NRHS 64 two chunks fetch ~10.5 GB per
-     * direction for a 4.4 GB factor, and still beat one 64-column chain
-     * (3.2 against 4.0 ms) or four 16-column ones (3.9 ms). */
+     * (shorter work per chain link).  Each chunk fetches the factor rows
+     * past the L2 itself: at N = 32768, NRHS 64 two chunks fetch ~10.5 GB
+     * per direction for a 4.4 GB factor, and still beat one 64-column
+     * chain (3.2 against 4.0 ms) or four 16-column ones (3.9 ms). */
     static const int snc = [] {
         const char *e = getenv("CHAMAMD_SOLVE_NC");
         const int v = e ? atoi(e) : 2;
