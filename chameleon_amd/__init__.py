@@ -37,6 +37,7 @@ SUCCESS = 0
 ERR_NOT_INITIALIZED = -101
 ERR_ILLEGAL_VALUE = -104
 COL_MAJOR = 102
+DIST_UNIFORM, DIST_SYMMETRIC, DIST_NORMAL = 201, 202, 203
 
 
 class ChameleonError(RuntimeError):
@@ -119,6 +120,8 @@ def _declare(L):
         "CHAMAMD_dposv_mixed_Tile": (i32, [i32, vp, vp, vp, i32, ip, dp]),
         "CHAMAMD_dposv_mixed_Tile_Async": (i32, [i32, vp, vp, vp, i32, ip, dp, vp, vp]),
         "CHAMAMD_dplgsy_cond_Tile": (i32, [dbl, i32, vp, u64]),
+        "CHAMAMD_Cond_Cache_Release": (i32, []),
+        "CHAMAMD_Set_Refinement_Target": (i32, [dbl]),
         "CHAMAMD_Backward_Error": (i32, [i32, vp, vp, vp, dp]),
         "CHAMAMD_Kernel_Stats": (i32, [i32, ctypes.POINTER(ctypes.c_longlong), dp, dp]),
         "CHAMAMD_Kernel_Stats_Reset": (i32, []),

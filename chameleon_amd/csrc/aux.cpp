@@ -7,7 +7,6 @@
 #include "internal.h"
 
 #include <cmath>
-#include <random>
 #include <vector>
 
 #include "chameleon_amd.h"
@@ -266,56 +265,6 @@ double CHAMELEON_dlansy(cham_normtype_t norm, cham_uplo_t uplo, int N, double *A
     double v = CHAMELEON_dlansy_Tile(norm, uplo, d);
     CHAMELEON_Desc_Destroy(&d);
     return v;
-}
-
-/* ---------------------------------------------- conditioned generator */
-int CHAMAMD_dplgsy_cond_Tile(double kappa, int nrefl, CHAM_desc_t *A, unsigned long long int seed)
-{
-    if (!ctx()) return CHAMELEON_ERR_NOT_INITIALIZED;
-    if (desc_check(A) != CHAMELEON_SUCCESS || cham_get_flttype(A->dtyp) != ChamRealDouble || A->mb != A->nb ||
-        A->m != A->n) {
-        error("CHAMAMD_dplgsy_cond_Tile", "A must be a square double descriptor with square tiles");
-        return CHAMELEON_ERR_ILLEGAL_VALUE;
-    }
-    if (!(kappa >= 1.0) || nrefl < 0) {
-        error("CHAMAMD_dplgsy_cond_Tile", "kappa must be >= 1 and nrefl >= 0");
-        return CHAMELEON_ERR_ILLEGAL_VALUE;
-    }
-    if (A->m == 0) return CHAMELEON_SUCCESS;
-    Context *c = ctx();
-    hipStream_t s = c->s_main;
-    mirror_prepare_overwrite(A, MIRROR_NORMAL, true);
-    Mirror *M = mirror_of(A);
-    const int NT = M->mt, ldt = M->ldt, N = A->m;
-    const size_t len = (size_t)NT * ldt;
-    double *dv = (double *)workspace2(3 * len * sizeof(double));
-    double *dw = dv + len, *dz = dw + len;
-    k::cond_init((double *)M->d, NT, ldt, N, kappa, s);
-    std::mt19937_64 gen(seed);
-    std::uniform_real_distribution<double> uni(-1.0, 1.0);
-    std::vector<double> v(len, 0.0), w(len), z(len, 0.0);
-    for (int r = 0; r < nrefl; ++r) {
-        double vv = 0.0;
-        for (int i = 0; i < N; ++i) {
-            v[i] = uni(gen);
-            vv += v[i] * v[i];
-        }
-        const double tau = 2.0 / vv;
-        CHX_HIP(hipMemcpyAsync(dv, v.data(), len * sizeof(double), hipMemcpyHostToDevice, s));
-        CHX_HIP(hipMemsetAsync(dw, 0, len * sizeof(double), s));
-        k::cond_gemv((const double *)M->d, NT, ldt, N, dv, dw, s);
-        CHX_HIP(hipMemcpyAsync(w.data(), dw, len * sizeof(double), hipMemcpyDeviceToHost, s));
-        CHX_HIP(hipStreamSynchronize(s));
-        double vtw = 0.0;
-        for (int i = 0; i < N; ++i) vtw += v[i] * w[i];
-        for (int i = 0; i < N; ++i) z[i] = tau * w[i] - 0.5 * tau * tau * vtw * v[i];
-        CHX_HIP(hipMemcpyAsync(dz, z.data(), len * sizeof(double), hipMemcpyHostToDevice, s));
-        k::cond_rank2((double *)M->d, NT, ldt, N, dv, dz, s);
-        CHX_HIP(hipStreamSynchronize(s));
-    }
-    mirror_mark_written(A);
-    A->sync = 1;
-    return CHAMELEON_SUCCESS;
 }
 
 /* ------------------------------------------------------------- lacpy */

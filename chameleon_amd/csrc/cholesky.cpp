@@ -353,6 +353,8 @@ int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequen
     SolvePlan plan = solve_plan((double *)MA->d, MA->mt, MA->ldt, nrp, st->d_info);
     solve_run(plan, (double *)MA->d, MA->mt, MA->ldt, (double *)MB->d, MB->nt, MB->wdt, st->d_info,
               2.0 * (double)A->m * (double)A->m * (double)B->n);
+    /* a solve wait that ran out is reported through the info flag */
+    st->pending_info = true;
     mirror_mark_written(B);
     B->sync = 1;
     return CHAMELEON_SUCCESS;

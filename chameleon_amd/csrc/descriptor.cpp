@@ -101,13 +101,17 @@ static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 static int desc_init(CHAM_desc_t *d, void *mat, cham_flttype_t dtyp, int mb, int nb, int bsiz, int lm, int ln,
                      int i, int j, int m, int n, blkaddr_fct_t get_blkaddr, blkldd_fct_t get_blkldd,
-                     blkrankof_fct_t get_rankof)
+                     blkrankof_fct_t get_rankof, void *get_rankof_arg)
 {
     static int nbdesc = 0;
     memset(d, 0, sizeof(*d));
     d->get_blktile = gettile;
     d->get_blkdim = getblkdim;
     d->get_rankof = get_rankof ? get_rankof : getrankof;
+    d->get_rankof_init = d->get_rankof;
+    d->get_rankof_init_arg = get_rankof_arg;
+    d->data_dist = nullptr;
+    d->dist_mstride = d->dist_nstride = 1;
     d->dtyp = dtyp;
     d->mb = mb; d->nb = nb; d->bsiz = bsiz;
     d->i = i; d->j = j; d->m = m; d->n = n;
@@ -171,7 +175,6 @@ int CHAMELEON_Desc_Create_User(CHAM_desc_t **descptr, void *mat, cham_flttype_t 
                                blkaddr_fct_t get_blkaddr, blkldd_fct_t get_blkldd, blkrankof_fct_t get_rankof,
                                void *get_rankof_arg)
 {
-    (void)get_rankof_arg;
     if (!ctx()) {
         error("CHAMELEON_Desc_Create", "CHAMELEON not initialized");
         return CHAMELEON_ERR_NOT_INITIALIZED;
@@ -186,7 +189,8 @@ int CHAMELEON_Desc_Create_User(CHAM_desc_t **descptr, void *mat, cham_flttype_t 
     }
     CHAM_desc_t *d = (CHAM_desc_t *)calloc(1, sizeof(CHAM_desc_t));
     if (!d) return CHAMELEON_ERR_OUT_OF_RESOURCES;
-    int rc = desc_init(d, mat, dtyp, mb, nb, bsiz, lm, ln, i, j, m, n, get_blkaddr, get_blkldd, get_rankof);
+    int rc = desc_init(d, mat, dtyp, mb, nb, bsiz, lm, ln, i, j, m, n, get_blkaddr, get_blkldd, get_rankof,
+                       get_rankof_arg);
     if (rc == CHAMELEON_SUCCESS) rc = desc_check(d);
     if (rc != CHAMELEON_SUCCESS) {
         if (d->alloc_mat) free(d->mat);

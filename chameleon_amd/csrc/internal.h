@@ -34,6 +34,7 @@ void fatal_hip(hipError_t e, const char *what, const char *file, int line);
 #define PANEL_STALLED (-(1 << 30))
 #define CHX_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) ::chx::fatal_hip(_e, #call, __FILE__, __LINE__); } while (0)
 
+void latms_release();      /* frees the conditioned generator's kept factor */
 void error(const char *func, const char *msg);
 void warning(const char *func, const char *msg);
 
@@ -58,6 +59,7 @@ struct Context {
     int translation = ChamInPlace;
     int lookahead = 1;
     int fwd_tail = -1;            /* dposv: tile rows solved forward after potrf (-1: chosen) */
+    double ir_target = 0.0;       /* mixed solve: backward-error target (0: LAPACK's test)  */
     int autotuning = 1;
     int warnings = 1;
     int nworkers = 1;
@@ -84,6 +86,7 @@ struct Context {
     unsigned *panel_ctr = nullptr;  /* ticket / finished counters of the fused panel kernel */
     int panel = 1;                  /* fused panel kernel (CHAMAMD_PANEL=0: POTRF + TRSM)   */
     int pairs = 1;                  /* paired bulk updates (CHAMAMD_PAIRS=0: one per step)  */
+    void *latms_cache = nullptr;    /* orthogonal factor kept by the conditioned generator   */
     long long st_launches[CHAMAMD_KERNEL_NCLASS] = {};
     double st_ms[CHAMAMD_KERNEL_NCLASS] = {};
     double st_flops[CHAMAMD_KERNEL_NCLASS] = {};
@@ -242,10 +245,17 @@ void axpy(double *Y, const double *X, size_t n, hipStream_t s);
 int copy_lower_norm(const double *S, double *W, int NT, int ldt, int mb, int N, double *part, double *rowsum,
                     hipStream_t s);
 size_t copy_lower_norm_scratch(int NT, int ldt);     /* doubles of `part` */
-/* cond.hip */
-void cond_init(double *A, int NT, int ldt, int N, double kappa, hipStream_t s);
-void cond_gemv(const double *A, int NT, int ldt, int N, const double *v, double *w, hipStream_t s);
-void cond_rank2(double *A, int NT, int ldt, int N, const double *v, const double *z, hipStream_t s);
+/* latms.hip: square tile mirrors of NT x NT tiles, ldt x ldt, mb valid rows per tile */
+void gauss(double *G, int NT, int ldt, int mb, int N, unsigned long long seed, hipStream_t s);
+void transpose_full(const double *src, double *dst, int NT, int ldt, hipStream_t s);
+void scale_cols(const double *Q, double *W, int NT, int ldt, const double *sc, hipStream_t s);
+void symmetrize(double *A, int NT, int ldt, hipStream_t s);
+void add_diag(double *A, int NT, int ldt, int mb, int N, double shift, hipStream_t s);
+void sumsq_tiles(const double *A, int ntiles, int ldt, double *part, hipStream_t s);
+/* C(m,n) = beta C(m,n) + alpha sum_{k0<=k<k1} A(m,k) B(n,k)^T over the tiles
+ * r0 <= m < r0+nr, c0 <= n < c0+nc, or (lower) m >= n of that square */
+void tile_gemm_nt(const double *A, int amt, const double *B, int bmt, double *C, int cmt, int ldt, int k0, int k1,
+                  int r0, int nr, int c0, int nc, int lower, double alpha, double beta, hipStream_t s);
 /* solve.hip */
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
                  int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s, int s0 = 0);

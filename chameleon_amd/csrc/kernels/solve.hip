@@ -204,7 +204,6 @@ struct StripArgs {
     int s0;               /* forward: first strip solved (the rows above are solved and
                            * already subtracted from the right-hand sides)           */
     int trans, epoch;
-    int pub;              /* 1: publish by coherent stores + completion, 0: release fence */
     const int *info;
 };
 
@@ -225,6 +224,9 @@ __device__ __forceinline__ void load_op4(const double *M, int ld, bool vec, int 
 }
 
 #define SR 64            /* strip rows */
+/* polls of one flag wait before it gives up: about a second at ~0.3 us per
+ * poll, far beyond any chain (a full direction at N = 32768 is ~3 ms) */
+#define STRIP_SPIN_LIMIT (1 << 22)
 template <int NC>
 __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
 {
@@ -276,8 +278,14 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
             if (fwd ? j <= ready : j >= ready) return;
             /* the lanes poll the flags of the next 64 strips; published
              * strips form a prefix of the chain, so the run of set flags
-             * starting at j is all that may be consumed, behind one acquire */
-            for (;;) {
+             * starting at j is all that may be consumed, behind one acquire.
+             * The wait is bounded: when the info flag turns nonzero (the
+             * factorisation failed, or another wait ran out) or the poll
+             * limit is reached (PANEL_STALLED recorded in info), the wave
+             * stops waiting and goes on with whatever the blocks hold, so
+             * every workgroup still reaches its barriers and the grid
+             * drains; the host reports the info value, not the result. */
+            for (int polls = 0;; ++polls) {
                 const int jj = fwd ? j + lane : j - lane;
                 const bool inr = fwd ? jj < s : jj > s;
                 const bool set =
@@ -287,6 +295,15 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
                     const int run = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
                     ready = fwd ? j + run - 1 : j - run + 1;
                     break;
+                }
+                if ((polls & 63) == 63) {
+                    const int inf = __hip_atomic_load(a.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (inf == 0 && polls >= STRIP_SPIN_LIMIT && lane == 0)
+                        atomicCAS((int *)a.info, 0, PANEL_STALLED);
+                    if (inf != 0 || polls >= STRIP_SPIN_LIMIT) {
+                        ready = fwd ? a.S : -1;      /* no further waits in this task */
+                        break;
+                    }
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
@@ -380,13 +397,14 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
                 for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv, Xs[p * NCOL + c * 16 + li], x[c]);
             }
         }
-        if (a.pub) {
+        {
             /* publication by completion: X_s is written with device-scope
              * (coherent, sc1) stores, which reach the memory side past this
              * XCD's L2 by themselves; once every wave has waited for its
              * stores to complete (vmcnt 0) and the workgroup has met at the
              * barrier, the flag store follows them.  A release fence would
-             * write back the whole L2 for the same guarantee. */
+             * write back the whole L2 for the same guarantee (measured
+             * slower: dpotrs at N = 32768 3.90 against 4.13 ms, NRHS 1). */
 #pragma unroll
             for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -396,18 +414,6 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
             __builtin_amdgcn_s_waitcnt(0x0F70);   /* vmcnt(0), expcnt and lgkmcnt not waited for */
             __syncthreads();
             if (tid == 0) __hip_atomic_store(&flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp] = x[c][e];
-            /* publication: the barrier completes every wave's stores, then one
-             * device-scope release by one thread covers them all */
-            __syncthreads();
-            if (tid == 0) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __hip_atomic_store(&flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
         }
     }
 }
@@ -442,22 +448,13 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
     if (a.s0 >= a.S) return 0;
     a.trans = trans;
     a.info = info;
-    static const int pub = [] {
-        const char *e = getenv("CHAMAMD_SOLVE_PUB");
-        return e ? atoi(e) : 1;
-    }();
-    a.pub = pub;
-    /* groups of up to 64 columns per launch; a group is split into chunks
-     * of 16 * SOLVE_NC columns that run as separate chains side by side
+    /* groups of up to 64 columns per launch; a full group is split into
+     * two chunks of 32 columns that run as separate chains side by side
      * (shorter work per chain link).  Each chunk fetches the factor rows
      * past the L2 itself: at N = 32768, NRHS 64 two chunks fetch ~10.5 GB
      * per direction for a 4.4 GB factor, and still beat one 64-column
      * chain (3.2 against 4.0 ms) or four 16-column ones (3.9 ms). */
-    static const int snc = [] {
-        const char *e = getenv("CHAMAMD_SOLVE_NC");
-        const int v = e ? atoi(e) : 2;
-        return (v == 1 || v == 2 || v == 4) ? v : 2;
-    }();
+    const int snc = 2;
     int used = 0;
     for (int c0 = 0; c0 < nrp; c0 += 64) {
         const int cols = (nrp - c0 >= 64) ? 64 : nrp - c0;

@@ -9,6 +9,9 @@
  *   - R = B - A X in FP64; column j has converged when
  *       ||R_j||_inf <= ||X_j||_inf * ||A||_inf * eps * sqrt(N);
  *   - otherwise X += A_lp^-1 R, at most ITERMAX = 30 times;
+ *   - with a refinement target t > 0 (CHAMAMD_Set_Refinement_Target) the
+ *     test is instead max_j ||R_j||_inf / (||A||_inf ||X_j||_inf N eps) <= t,
+ *     so that a solve can be driven to the FP64 solve's backward error;
  *   - on a reduced-precision factorisation failure (ITER = -3) or no
  *     convergence (ITER = -31) the plain FP64 factorisation is used and A
  *     holds its factor on return; otherwise A is unchanged.
@@ -214,6 +217,9 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
                     const double d = anrm * xn[pj] * N * eps;
                     worst = std::fmax(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
                 }
+                /* with a target, refinement goes on (converged or not by the
+                 * test above) until every column's backward error is within it */
+                if (c->ir_target > 0.0) ok = (worst <= c->ir_target);
                 if (ok) {
                     converged = true;
                     it_count = it;
@@ -223,6 +229,10 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
                 solve_bt(sp, Aw, NT, ldt, RT, nrp, st->d_info);
                 k::axpy(XT, RT, vb, s);
             }
+            /* a strip-solve wait that ran out leaves its code in the info
+             * flag: the refinement result is then not trusted */
+            st->pending_info = true;
+            if (sequence_wait_internal(seq) != CHAMELEON_SUCCESS) converged = false;
             if (converged) {
                 k::rhs_transpose((double *)MX->d, XT, NT, ldt, MX->wdt, MX->nt, 0, s);
                 CHX_HIP(hipStreamSynchronize(s));
@@ -306,6 +316,20 @@ int CHAMAMD_dposv_mixed_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t
     request->status = CHAMELEON_SUCCESS;
     int rc = mixed_checked(fn, uplo, A, B, X, prec, iter, berr);
     return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
+}
+
+int CHAMAMD_Set_Refinement_Target(double berr)
+{
+    if (!ctx()) {
+        error("CHAMAMD_Set_Refinement_Target", "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    if (!(berr >= 0.0)) {
+        error("CHAMAMD_Set_Refinement_Target", "the target must be >= 0");
+        return CHAMELEON_ERR_ILLEGAL_VALUE;
+    }
+    ctx()->ir_target = berr;
+    return CHAMELEON_SUCCESS;
 }
 
 int CHAMAMD_Backward_Error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t *B, double *berr)

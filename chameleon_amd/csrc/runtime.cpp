@@ -362,6 +362,7 @@ int __chameleon_finalize(void)
         return CHAMELEON_ERR_NOT_INITIALIZED;
     }
     CHX_HIP(hipDeviceSynchronize());
+    latms_release();
     for (hipEvent_t e : c->events) CHX_HIP(hipEventDestroy(e));
     stats_fold();
     for (hipEvent_t e : c->timing_pool) CHX_HIP(hipEventDestroy(e));

@@ -74,6 +74,12 @@ typedef enum chameleon_mtxtype_e {
     ChamGeneral = 231, ChamSymmetric = 232, ChamHermitian = 233, ChamTriangular = 234
 } cham_mtxtype_t;
 
+/* distributions of the random matrix generators (reference
+ * include/chameleon/constants.h:211-215) */
+typedef enum chameleon_dist_e {
+    ChamDistUniform = 201, ChamDistSymmetric = 202, ChamDistNormal = 203
+} cham_dist_t;
+
 typedef enum chameleon_access_e { ChamR = 1, ChamW = 2, ChamRW = 3 } cham_access_t;
 
 typedef enum chameleon_translation_e { ChamInPlace = 1, ChamOutOfPlace = 2 } cham_translation_t;

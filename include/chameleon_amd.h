@@ -71,13 +71,31 @@ int CHAMAMD_dposv_mixed_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t
                                    int *iter, double *berr, RUNTIME_sequence_t *sequence,
                                    RUNTIME_request_t *request);
 
-/* Symmetric positive definite matrix with eigenvalues kappa^(-i/(N-1)),
- * i < N, generated on the device: A = Q diag(lambda) Q^T with Q the
- * product of `nrefl` random Householder reflectors (seeded).  Conditioned
- * test matrices in the role of the reference's dlatms generator
- * (compute/pzlatms.c, mode 3 spectrum of core_zlatm1.c), at O(nrefl N^2)
- * cost.  The full matrix (both triangles) is written. */
-int CHAMAMD_dplgsy_cond_Tile(double kappa, int nrefl, CHAM_desc_t *A, unsigned long long int seed);
+/* Symmetric positive definite matrix with eigenvalues lambda_i = alpha^i,
+ * alpha = kappa^(-1/(N-1)), i < N, generated on the device:
+ * A = Q diag(lambda) Q^T with Q the orthogonal factor of the QR
+ * factorisation of a seeded random N x N matrix G -- the construction of
+ * the reference's dlatms generator (compute/pzlatms.c:194-239) with the
+ * mode-3 spectrum of coreblas/compute/core_zlatm1.c:172-180.
+ * dist = ChamDistSymmetric: G = dplrnt(N x N) with the seed advanced 2N
+ * states (pzlatms.c:139), the reference's choice; ChamDistNormal: G has
+ * N(0,1) entries (Box-Muller on consecutive LCG values), Q is Haar
+ * distributed.  Q is formed by shifted CholeskyQR3 on the device (about
+ * 7 N^3 flops) and kept for further calls with the same N, tiling, seed
+ * and dist; each further kappa costs one N^3 product.  The full matrix
+ * (both triangles) is written. */
+int CHAMAMD_dplgsy_cond_Tile(double kappa, int dist, CHAM_desc_t *A, unsigned long long int seed);
+/* frees the orthogonal factor kept by CHAMAMD_dplgsy_cond_Tile (N^2 doubles) */
+int CHAMAMD_Cond_Cache_Release(void);
+
+/* Stopping test of the refinement.  berr = 0 (the default): LAPACK
+ * dsposv's test ||R_j||_inf <= ||X_j||_inf ||A||_inf eps sqrt(N) per
+ * column.  berr > 0: refine until every column's backward error
+ * ||R_j||_inf / (||A||_inf ||X_j||_inf N eps) is at most berr (still at most
+ * 30 steps, then the FP64 fallback), e.g. to reach the backward error of an
+ * FP64 solve of the same system.  Returns CHAMELEON_SUCCESS,
+ * CHAMELEON_ERR_ILLEGAL_VALUE (berr < 0) or CHAMELEON_ERR_NOT_INITIALIZED. */
+int CHAMAMD_Set_Refinement_Target(double berr);
 
 /* The same backward error for any X, A symmetric given by its `uplo`
  * triangle, computed on the device in FP64. */

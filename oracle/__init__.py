@@ -142,6 +142,20 @@ def find_openblas():
 _blas_handle = None
 
 
+def blas_config():
+    """Build string of the OpenBLAS that find_openblas() picks (its
+    ``scipy_openblas_get_config``), or None."""
+    path = find_openblas()
+    if path is None:
+        return None
+    h = ctypes.CDLL(path)
+    fn = getattr(h, "scipy_openblas_get_config", None)
+    if fn is None:
+        return None
+    fn.restype = ctypes.c_char_p
+    return fn().decode(errors="replace")
+
+
 def use_blas(enable=True):
     """Route the oracle's tile kernels through host OpenBLAS (single-threaded
     inside each task, as coreblas assumes) or back to the textbook loops."""
@@ -200,3 +214,69 @@ def check_solve(A0, X, B):
 def rel_frobenius(X, Y):
     d = np.linalg.norm(Y)
     return np.linalg.norm(X - Y) / (d if d > 0 else 1.0)
+
+
+# ------------------------------------------------------- conditioned SPD
+# Restatement of the conditioned generator CHAMAMD_dplgsy_cond_Tile: the
+# construction of the reference's chameleon_pzlatms (compute/pzlatms.c:
+# 194-239: U <- random matrix, U <- qr(U), A <- U D U^T) with the mode-3
+# spectrum of coreblas/compute/core_zlatm1.c:172-180.  Q comes from numpy's
+# Householder QR here (LAPACK dgeqrf/dorgqr), an algorithm independent of
+# the device's CholeskyQR3; both give the same Q up to column signs, which
+# cancel in Q D Q^T.
+_RND_A = np.uint64(6364136223846793005)
+_RND_C = np.uint64(1)
+
+
+def lcg_states(positions, seed):
+    """LCG state after `positions[k]` steps from `seed` (coreblas/include/
+    coreblas/random.h:32-52, binary powering of the affine map), vectorised."""
+    n = np.asarray(positions, dtype=np.uint64).copy()
+    x = np.full(n.shape, np.uint64(seed), dtype=np.uint64)
+    a, c = _RND_A, _RND_C
+    with np.errstate(over="ignore"):
+        for _ in range(64):
+            bit = (n & np.uint64(1)).astype(bool)
+            x = np.where(bit, a * x + c, x)
+            c = c * (a + np.uint64(1))
+            a = a * a
+            n = n >> np.uint64(1)
+    return x
+
+
+def gauss_matrix(n, seed):
+    """N(0,1) matrix of the generator's ChamDistNormal rule: element (i, j)
+    takes the LCG states x1, x2 after 2p and 2p + 1 steps, p = i + j n,
+    u = ((x >> 11) + 1/2) 2^-53, G = sqrt(-2 ln u1) cos(2 pi u2)."""
+    i, j = np.meshgrid(np.arange(n, dtype=np.uint64), np.arange(n, dtype=np.uint64), indexing="ij")
+    p = i + j * np.uint64(n)
+    x1 = lcg_states(2 * p, seed)
+    with np.errstate(over="ignore"):
+        x2 = _RND_A * x1 + _RND_C
+    u1 = ((x1 >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+    u2 = ((x2 >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+    return np.asfortranarray(np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2))
+
+
+def latm1_mode3(n, kappa):
+    """D of core_zlatm1.c:172-180 (mode 3): D[0] = 1, D[i] = alpha^i,
+    alpha = kappa^(-1/(n-1))."""
+    if n <= 1:
+        return np.ones(n)
+    alpha = kappa ** (-1.0 / (n - 1))
+    return np.array([alpha ** i for i in range(n)])
+
+
+def latms(kappa, n, seed, dist=203):
+    """A = Q diag(D) Q^T, Q = qr(G); G Gaussian (dist 203, ChamDistNormal) or
+    Chameleon's dplrnt matrix with the seed advanced 2n states (dist 202,
+    pzlatms.c:139,197)."""
+    if dist == 203:
+        G = gauss_matrix(n, seed)
+    else:
+        G = dplrnt(n, n, rnd64_jump(2 * n, seed))
+    Q, _ = np.linalg.qr(G)
+    D = latm1_mode3(n, kappa)
+    A = (Q * D) @ Q.T
+    A = 0.5 * (A + A.T)
+    return np.asfortranarray(A), D
