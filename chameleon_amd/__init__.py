@@ -122,6 +122,9 @@ def _declare(L):
         "CHAMAMD_dplgsy_cond_Tile": (i32, [dbl, i32, vp, u64]),
         "CHAMAMD_Cond_Cache_Release": (i32, []),
         "CHAMAMD_Set_Refinement_Target": (i32, [dbl]),
+        "CHAMAMD_Set_Mixed_Scaling": (i32, [i32]),
+        "CHAMAMD_Set_Tile_Accuracy": (i32, [dbl]),
+        "CHAMAMD_Mixed_Tile_Counts": (i32, [ctypes.POINTER(ctypes.c_longlong)]),
         "CHAMAMD_Backward_Error": (i32, [i32, vp, vp, vp, dp]),
         "CHAMAMD_Kernel_Stats": (i32, [i32, ctypes.POINTER(ctypes.c_longlong), dp, dp]),
         "CHAMAMD_Kernel_Stats_Reset": (i32, []),

@@ -76,6 +76,7 @@ void potrf_run(const PotrfPlan &p)
      * columns takes the panels of steps k and k+1 in one pass */
     const bool pairs = c->pairs && c->lookahead > 0 && NT >= 4 && (p.lp_mode < 0 || p.P[3] != nullptr);
     auto pbuf = [&](int k) { return pairs ? p.P[k % 4] : p.P[k & 1]; };
+    auto pbuf2 = [&](int k) { return pairs ? p.P2[k % 4] : p.P2[k & 1]; };
     auto update = [&](int k, int n0, int n1, hipStream_t s, int kp = 1) {
         n1 = std::min(n1, NT);
         if (n0 >= NT || n1 <= n0) return;
@@ -83,9 +84,20 @@ void potrf_run(const PotrfPlan &p)
          * columns; they are timed apart from the bulk update */
         const int cls = (s == sm && c->lookahead > 0) ? CHAMAMD_KERNEL_COLUPD : CHAMAMD_KERNEL_UPDATE;
         timed(cls, kp * update_flops(k, n0, n1), s, [&] {
-            if (p.lp_mode < 0) k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s, kp);
-            else k::trailing_update_lp(p.A, pbuf(k), kp == 2 ? pbuf(k + 1) : nullptr, p.lp_mode, NT, ldt, n0, n1, NT,
-                                       p.d_info, s);
+            if (p.lp_mode < 0) {
+                k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s, kp);
+            } else if (p.lp_mode == CHAMAMD_PREC_ADAPTIVE) {
+                /* every tile in the precision the map gives it: three
+                 * launches over the same tiles, each skipping the others' */
+                k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s, kp, p.pmap);
+                k::trailing_update_lp(p.A, pbuf(k), kp == 2 ? pbuf(k + 1) : nullptr, CHAMAMD_PREC_FP32, NT, ldt, n0,
+                                      n1, NT, p.d_info, s, p.pmap);
+                k::trailing_update_lp(p.A, pbuf2(k), kp == 2 ? pbuf2(k + 1) : nullptr, CHAMAMD_PREC_BF16, NT, ldt, n0,
+                                      n1, NT, p.d_info, s, p.pmap);
+            } else {
+                k::trailing_update_lp(p.A, pbuf(k), kp == 2 ? pbuf(k + 1) : nullptr, p.lp_mode, NT, ldt, n0, n1, NT,
+                                      p.d_info, s);
+            }
         });
     };
     /* POTRF(k) runs next to the bulk update of step k-1 (columns k+1..).
@@ -103,6 +115,18 @@ void potrf_run(const PotrfPlan &p)
         return (S * (S + 1) / 2) * bpt >= 2LL * ncu ? 4 : 8;
     };
     const bool fused = c->panel && ldt % 64 == 0;
+    /* the tiles below the diagonal of panel k in the precision of the
+     * trailing products */
+    auto demote = [&](int k) {
+        if (p.lp_mode == CHAMAMD_PREC_FP8) {
+            k::panel_demote_fp8(p.A, NT, ldt, k, k + 1, NT, pbuf(k), p.d_info, sm);
+        } else if (p.lp_mode == CHAMAMD_PREC_ADAPTIVE) {
+            k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf(k), CHAMAMD_PREC_FP32, sm);
+            k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf2(k), CHAMAMD_PREC_BF16, sm);
+        } else {
+            k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf(k), p.lp_mode, sm);
+        }
+    };
     auto panel = [&](int k) {
         const int nvalid = (k == NT - 1) ? p.nvalid_last : p.nb;
         double *dk = dinv + (size_t)k * dper;
@@ -119,7 +143,7 @@ void potrf_run(const PotrfPlan &p)
                 k::panel_factor(tile(k, k), tile(std::min(k + 1, NT - 1), k), (long long)ts, NT - k - 1, ldt, dk,
                                 flags, ctr, epoch, p.d_info, p.nb * k, nvalid, sm);
             });
-            if (k + 1 < NT && p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf(k), p.lp_mode, sm);
+            if (k + 1 < NT && p.lp_mode >= 0) demote(k);
             return;
         }
         const int waves = potrf_waves(k);
@@ -131,7 +155,7 @@ void potrf_run(const PotrfPlan &p)
             timed(CHAMAMD_KERNEL_TRSM, f, sm, [&] {
                 k::trsm_right_lower_trans(tile(k + 1, k), NT - k - 1, ts, ldt, tile(k, k), dk, p.d_info, sm);
             });
-            if (p.lp_mode >= 0) k::panel_demote(p.A, NT, ldt, k, k + 1, NT, pbuf(k), p.lp_mode, sm);
+            if (p.lp_mode >= 0) demote(k);
         }
     };
     if (c->lookahead <= 0 || NT <= 2) {

@@ -32,6 +32,8 @@ void fatal_hip(hipError_t e, const char *what, const char *file, int line);
 /* info value left by a kernel whose bounded wait for another workgroup ran
  * out (an internal error, never a property of the matrix) */
 #define PANEL_STALLED (-(1 << 30))
+/* info value of a low-precision demotion that met a non-finite entry */
+#define LP_RANGE (-(1 << 29))
 #define CHX_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) ::chx::fatal_hip(_e, #call, __FILE__, __LINE__); } while (0)
 
 void latms_release();      /* frees the conditioned generator's kept factor */
@@ -60,6 +62,9 @@ struct Context {
     int lookahead = 1;
     int fwd_tail = -1;            /* dposv: tile rows solved forward after potrf (-1: chosen) */
     double ir_target = 0.0;       /* mixed solve: backward-error target (0: LAPACK's test)  */
+    int mixed_scaling = 1;        /* mixed solve: symmetric diagonal scaling before demotion */
+    double tile_accuracy = 1e-12; /* per-tile precision choice: requested accuracy u_high    */
+    long long tile_counts[3] = {0, 0, 0};   /* last per-tile choice: FP64, FP32, BF16 tiles */
     int autotuning = 1;
     int warnings = 1;
     int nworkers = 1;
@@ -150,6 +155,10 @@ struct PotrfPlan {
      * operands, demoted panel alternating between P[0] and P[1] */
     int lp_mode = -1;
     void *P[4] = {nullptr, nullptr, nullptr, nullptr};   /* panel k in P[k % 2], or P[k % 4] with paired updates */
+    /* per-tile precision (lp_mode = CHAMAMD_PREC_ADAPTIVE): FP32 panels in P,
+     * BF16 panels in P2, the precision of tile (m,n) at pmap[m + n*NT] */
+    void *P2[4] = {nullptr, nullptr, nullptr, nullptr};
+    const unsigned char *pmap = nullptr;
     /* called where the main stream is about to wait for a bulk update
      * (with lookahead it usually has time to spare there), when panels
      * 0..k are issued on it: work the callback issues on the main stream
@@ -215,8 +224,11 @@ void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, i
  * *ncol to the launch mode: the column count, or 0 for the whole triangle
  * n0 .. NT-1 (n1 >= NT and more than one column). */
 int update_pairs(int n0, int n1, int NT, int *ncol);
+/* pmap (NT x NT bytes, tile (m,n) at m + n*NT, or null): when given, only
+ * the tiles marked 0 (FP64) are updated; the low-precision updates take
+ * the tiles marked 1 (FP32) and 2 (BF16) */
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
-                     int kp = 1);
+                     int kp = 1, const unsigned char *pmap = nullptr);
 void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
 void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, const int *nvalid, double *dinv,
                 const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s);
@@ -231,8 +243,17 @@ void tile_pad_diag(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, 
                    hipStream_t s);
 /* mixed.hip */
 void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int mode, hipStream_t s);
+/* FP8: tiles m0..m1-1 of panel k as E4M3 bytes with a power-of-two scale
+ * per tile; a tile with a non-finite entry sets *info = LP_RANGE */
+void panel_demote_fp8(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int *info, hipStream_t s);
+/* symmetric diagonal scaling: sc[i] = a_ii^-1/2 over the NT*ldt padded rows,
+ * lower tiles of A scaled by sc_i sc_j; scale_bt: rows of transposed
+ * right-hand sides (element (c, r) at c + r*nrp) times sc[r] */
+void diag_scaling(double *A, int NT, int ldt, double *sc, hipStream_t s);
+void scale_bt(double *BT, int nrp, int rows, const double *sc, hipStream_t s);
 void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int lmt, int ldt, int n0, int n1, int NT,
-                        const int *info, hipStream_t s);
+                        const int *info, hipStream_t s, const unsigned char *pmap = nullptr);
+void tile_sumsq_lower(const double *A, int NT, int ldt, double *ss, hipStream_t s);
 void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s);
 size_t residual_scratch(int NT, int ldt, int nrp);   /* doubles of `part` */

@@ -74,7 +74,8 @@ __device__ __forceinline__ void lstore_slice(double *lds, const v2d (&r)[L], int
  * segment.                                                                */
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update_kernel(
-    double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int ncol, int total_blocks, const int *info)
+    double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int ncol, int total_blocks, const int *info,
+    const unsigned char *pmap)
 {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NTH = NW * 64;
@@ -115,6 +116,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
         m = n0 + i;
         n = n0 + j;
     }
+    /* per-tile precision map: this launch updates the FP64 tiles only */
+    if (pmap && pmap[m + (size_t)n * NT] != 0) return;
     const int bi = b % tbm, bj = b / tbm;
     const int row0 = bi * BM, col0 = bj * BN;
     const bool diag = (m == n);
@@ -194,14 +197,14 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
 
 template <int BM, int BN, int WM, int WN>
 static void launch_update(double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int ncol, int npairs,
-                          const int *info, hipStream_t s)
+                          const int *info, hipStream_t s, const unsigned char *pmap)
 {
     const int bpt = (ldt / BM) * (ldt / BN);
     const long long total = (long long)npairs * bpt;
     if (total <= 0) return;
     constexpr int NTH = (BM / WM) * (BN / WN) * 64;
     hipLaunchKernelGGL((trailing_update_kernel<BM, BN, WM, WN>), dim3((unsigned)total), dim3(NTH), 0, s, A, lmt,
-                       ldt, k, kp, n0, NT, ncol, (int)total, info);
+                       ldt, k, kp, n0, NT, ncol, (int)total, info, pmap);
 }
 
 int update_pairs(int n0, int n1, int NT, int *ncol)
@@ -216,7 +219,7 @@ int update_pairs(int n0, int n1, int NT, int *ncol)
 }
 
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
-                     int kp)
+                     int kp, const unsigned char *pmap)
 {
     if (n0 >= NT || n1 <= n0) return;
     int ncol = 0;
@@ -237,13 +240,13 @@ void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT,
     }();
     const bool small_cols = use_small && ncol > 0 && (long long)npairs * (ldt / 128) * (ldt / 128) < 2LL * ncu;
     if (ldt % 128 == 0 && !small_cols)
-        launch_update<128, 128, 32, 64>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
+        launch_update<128, 128, 32, 64>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s, pmap);
     else if (ldt % 64 == 0)
-        launch_update<64, 64, 32, 32>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
+        launch_update<64, 64, 32, 32>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s, pmap);
     else if (ldt % 32 == 0)
-        launch_update<32, 32, 16, 16>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
+        launch_update<32, 32, 16, 16>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s, pmap);
     else
-        launch_update<16, 16, 16, 16>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s);
+        launch_update<16, 16, 16, 16>(A, lmt, ldt, k, kp, n0, NT, ncol, npairs, info, s, pmap);
 }
 
 /* ================================================================== */

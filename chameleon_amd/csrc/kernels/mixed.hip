@@ -12,11 +12,16 @@
  *   LP_BF16   BF16 operands, v_mfma_f32_16x16x32_bf16
  *   LP_BF16X3 each operand split x = hi + lo (two BF16), three products
  *             hi*hi + hi*lo + lo*hi, relative error ~2^-16 per product
+ *   LP_FP8    OCP E4M3 operands with one power-of-two scale per tile,
+ *             v_mfma_scale_f32_16x16x128_f8f6f4 (the scale applied by the
+ *             MFMA)
  * Also: the FP64 symmetric residual R = B - A X and column max-norms used by
  * the refinement loop.
  */
 #include "../internal.h"
 #include "mfma.h"
+
+#include <algorithm>
 
 namespace chx {
 namespace k {
@@ -113,7 +118,7 @@ __device__ __forceinline__ void decode_pair(int pair, int n0, int ncol, int NT, 
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kernel(
     double *A, const float *P, const float *P2, int lmt, int ldt, int n0, int ncol, int total_blocks,
-    const int *info)
+    const int *info, const unsigned char *pmap)
 {
     constexpr int NW = (BM / WM) * (BN / WN), NTH = NW * 64, BK = 32;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -127,6 +132,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
     const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
     int m, n;
     decode_pair(bid / bpt, n0, ncol, lmt, m, n);   /* the tile grid is lmt x lmt, lmt = NT */
+    if (pmap && pmap[m + (size_t)n * lmt] != 1) return;   /* per-tile precision map: FP32 tiles */
     const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
     const bool diag = (m == n);
     if (diag && col0 >= row0 + BM) return;
@@ -218,7 +224,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
 template <int SPLIT>
 __global__ __launch_bounds__(512, 4) void update_bf16_kernel(double *A, const unsigned short *P,
                                                              const unsigned short *P2, int NT, int lmt, int ldt,
-                                                             int n0, int ncol, int total_blocks, const int *info)
+                                                             int n0, int ncol, int total_blocks, const int *info,
+                                                             const unsigned char *pmap)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 64, LDK = BK;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -233,6 +240,7 @@ __global__ __launch_bounds__(512, 4) void update_bf16_kernel(double *A, const un
     const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
     int m, n;
     decode_pair(bid / bpt, n0, ncol, NT, m, n);
+    if (pmap && pmap[m + (size_t)n * NT] != 2) return;   /* per-tile precision map: BF16 tiles */
     const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
     const bool diag = (m == n);
     if (diag && col0 >= row0 + BM) return;
@@ -338,7 +346,8 @@ __global__ __launch_bounds__(512, 4) void update_bf16_kernel(double *A, const un
 template <int SPLIT>
 __global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const unsigned short *P,
                                                               const unsigned short *P2, int NT, int lmt, int ldt,
-                                                              int n0, int ncol, int total_blocks, const int *info)
+                                                              int n0, int ncol, int total_blocks, const int *info,
+                                                              const unsigned char *pmap)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 32;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -352,6 +361,7 @@ __global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const u
     const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
     int m, n;
     decode_pair(bid / bpt, n0, ncol, NT, m, n);
+    if (pmap && pmap[m + (size_t)n * NT] != 2) return;
     const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
     const bool diag = (m == n);
     if (diag && col0 >= row0 + BM) return;
@@ -446,8 +456,297 @@ __global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const u
             }
 }
 
+/* ------------------------------------------------------------------ */
+/* FP8 (OCP E4M3: 4 exponent bits, bias 7, 3 mantissa bits, largest finite
+ * 448, subnormals from 2^-9) operands with one power-of-two scale per
+ * demoted tile.  Tile m of panel k is stored as bytes e4m3(x 2^e_m),
+ * row-major (k contiguous), e_m = 7 - floor(log2 max|tile|) so the scaled
+ * maximum lies in [128, 256); the update kernel hands 2^-e_m to the MFMA as
+ * its per-operand E8M0 scale (v_mfma_scale_f32_16x16x128_f8f6f4), so the
+ * products come out unscaled in FP32.  Panel buffer layout: NT tiles of
+ * ldt^2 bytes, then NT tile maxima (float bits), then NT exponents e_m.  */
+
+/* round-to-nearest-even into E4M3; |x| >= 464 saturates to 448 */
+__device__ __forceinline__ unsigned char e4m3_rne(double x)
+{
+    const unsigned char sign = (x < 0.0) ? 0x80 : 0x00;
+    const double a = fabs(x);
+    if (!(a < 464.0)) return sign | 0x7E;
+    if (a < 0x1p-6) return sign | (unsigned char)rint(a * 0x1p9);   /* 8 encodes 2^-6 */
+    int E = ilogb(a);
+    double r = rint(ldexp(a, 3 - E));
+    if (r == 16.0) {
+        r = 8.0;
+        ++E;
+    }
+    return sign | (unsigned char)(((E + 7) << 3) | ((int)r - 8));
+}
+
+__global__ __launch_bounds__(256) void tile_absmax_kernel(const double *A, int NT, int ldt, int k, int m0,
+                                                           unsigned *maxbits)
+{
+    const size_t ts = (size_t)ldt * ldt;
+    const int m = m0 + blockIdx.z;
+    const double *T = A + ((size_t)k * NT + m) * ts;
+    const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    float mx = 0.f;
+    for (int jj = ty; jj < 32; jj += 8)
+        if (r0 + tx < ldt && c0 + jj < ldt) mx = fmaxf(mx, (float)fabs(T[(r0 + tx) + (size_t)(c0 + jj) * ldt]));
+    /* non-negative floats order like their bits; a NaN compares above inf */
+    unsigned u = __float_as_uint(mx);
+    if (mx != mx) u = 0x7fc00000u;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(u, off);
+        u = o > u ? o : u;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(&maxbits[m], u);
+}
+
+__global__ __launch_bounds__(256) void demote_fp8_kernel(const double *A, int NT, int ldt, int k, int m0,
+                                                          unsigned char *P, const unsigned *maxbits, int *ex,
+                                                          int *info)
+{
+    const size_t ts = (size_t)ldt * ldt;
+    const int m = m0 + blockIdx.z;
+    const double *T = A + ((size_t)k * NT + m) * ts;
+    const float mx = __uint_as_float(maxbits[m]);
+    int e = 0;
+    if (mx > 0.f && mx <= 3.0e38f) e = 7 - ilogbf(mx);
+    e = max(-126, min(126, e));
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        ex[m] = e;
+        if (!(mx <= 3.0e38f)) atomicCAS(info, 0, LP_RANGE);    /* inf / NaN: no FP8 scale fits */
+    }
+    unsigned char *D = P + (size_t)m * ts;
+    const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int jj = ty; jj < 32; jj += 8) {
+        const int i = r0 + jj, p = c0 + tx;
+        if (i >= ldt || p >= ldt) continue;
+        D[p + (size_t)i * ldt] = e4m3_rne(ldexp(T[i + (size_t)p * ldt], e));
+    }
+}
+
+void panel_demote_fp8(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int *info, hipStream_t s)
+{
+    if (m1 <= m0) return;
+    const size_t ts = (size_t)ldt * ldt;
+    unsigned *maxbits = (unsigned *)((unsigned char *)P + (size_t)NT * ts);
+    int *ex = (int *)(maxbits + NT);
+    zero_ints((int *)maxbits + m0, m1 - m0, s);
+    dim3 grid((ldt + 31) / 32, (ldt + 31) / 32, m1 - m0);
+    hipLaunchKernelGGL(tile_absmax_kernel, grid, dim3(256), 0, s, A, NT, ldt, k, m0, maxbits);
+    hipLaunchKernelGGL(demote_fp8_kernel, grid, dim3(256), 0, s, A, NT, ldt, k, m0, (unsigned char *)P, maxbits, ex,
+                       info);
+}
+
+typedef int v8i32 __attribute__((ext_vector_type(8)));
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+
+/* FP8-operand trailing update: 128 x 128 blocks, 8 wavefronts of 64 x 32,
+ * k in 128-byte slices (one 16x16x128 MFMA deep), LDS rows of 128 bytes
+ * with the 16-byte chunk q of row r at q ^ ((r >> 1) & 7) (the slice
+ * stores and the operand reads of 16 rows x 32 bytes are free of bank
+ * conflicts), next slice prefetched into registers.  Lane (li, lk)
+ * supplies bytes 32 lk .. 32 lk + 31 of its row for both operands (the
+ * same k assignment on both, so the sum over k is complete). */
+__global__ __launch_bounds__(512, 2) void update_fp8_kernel(double *A, const unsigned char *P,
+                                                             const unsigned char *P2, int NT, int lmt, int ldt,
+                                                             int n0, int ncol, int total_blocks, const int *info)
+{
+    constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 128;
+    constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
+    constexpr int LA = (BM * BK) / (16 * NTH), LB = (BN * BK) / (16 * NTH);
+    __shared__ unsigned char As[BM * BK];
+    __shared__ unsigned char Bs[BN * BK];
+    if (*info != 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int bid = xcd_remap_m(blockIdx.x, total_blocks);
+    const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
+    int m, n;
+    decode_pair(bid / bpt, n0, ncol, NT, m, n);
+    const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
+    const bool diag = (m == n);
+    if (diag && col0 >= row0 + BM) return;
+    const size_t ts = (size_t)ldt * ldt;
+    double *C = A + ((size_t)n * lmt + m) * ts;
+    const int wm = (wave % (BM / WM)) * WM, wn = (wave / (BM / WM)) * WN;
+    auto soff = [](int r, int q) { return r * BK + 16 * (q ^ ((r >> 1) & 7)); };
+    /* E8M0 descale bytes of each panel: 2^-e as 127 - e */
+    const int *ex1 = (const int *)(P + (size_t)NT * ts + (size_t)NT * 4);
+    const int sa1 = 127 - ex1[m], sb1 = 127 - ex1[n];
+    int sa2 = sa1, sb2 = sb1;
+    if (P2) {
+        const int *ex2 = (const int *)(P2 + (size_t)NT * ts + (size_t)NT * 4);
+        sa2 = 127 - ex2[m];
+        sb2 = 127 - ex2[n];
+    }
+    v4f acc[FM][FN];
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = zero4f();
+    const int K = (P2 ? 2 : 1) * ldt;
+    v4i32 ra[LA], rb[LB];
+    auto gload = [&](int kb2) {
+        const unsigned char *Pp = kb2 < ldt ? P : P2;
+        const int kb = kb2 < ldt ? kb2 : kb2 - ldt;
+        const unsigned char *Pm = Pp + (size_t)m * ts, *Pn = Pp + (size_t)n * ts;
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int e = (it * NTH + tid) * 16, r = e / BK, q = e % BK;
+            ra[it] = *reinterpret_cast<const v4i32 *>(Pm + (size_t)(row0 + r) * ldt + kb + q);
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int e = (it * NTH + tid) * 16, r = e / BK, q = e % BK;
+            rb[it] = *reinterpret_cast<const v4i32 *>(Pn + (size_t)(col0 + r) * ldt + kb + q);
+        }
+    };
+    gload(0);
+    for (int kb2 = 0; kb2 < K; kb2 += BK) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int e = (it * NTH + tid) * 16;
+            *reinterpret_cast<v4i32 *>(&As[soff(e / BK, (e % BK) / 16)]) = ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int e = (it * NTH + tid) * 16;
+            *reinterpret_cast<v4i32 *>(&Bs[soff(e / BK, (e % BK) / 16)]) = rb[it];
+        }
+        __syncthreads();
+        if (kb2 + BK < K) gload(kb2 + BK);
+        const int sa = kb2 < ldt ? sa1 : sa2, sb = kb2 < ldt ? sb1 : sb2;
+        v8i32 af[FM], bf[FN];
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm) {
+            const int r = wm + fm * 16 + li;
+            const v4i32 lo = *reinterpret_cast<const v4i32 *>(&As[soff(r, 2 * lk)]);
+            const v4i32 hi = *reinterpret_cast<const v4i32 *>(&As[soff(r, 2 * lk + 1)]);
+            af[fm] = v8i32{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) {
+            const int r = wn + fn * 16 + li;
+            const v4i32 lo = *reinterpret_cast<const v4i32 *>(&Bs[soff(r, 2 * lk)]);
+            const v4i32 hi = *reinterpret_cast<const v4i32 *>(&Bs[soff(r, 2 * lk + 1)]);
+            bf[fn] = v8i32{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+        /* first operand: the B rows (results land as consecutive rows of C) */
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+                acc[fm][fn] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(bf[fn], af[fm], acc[fm][fn], 0, 0, 0, sb,
+                                                                              0, sa);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int gi = row0 + wm + fm * 16 + li;
+                const int gj = col0 + wn + fn * 16 + 4 * lk + s;
+                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= (double)acc[fm][fn][s];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* Symmetric diagonal scaling of the working copy: s_i = 1 / sqrt(a_ii)
+ * (1 where a_ii <= 0: the factorisation reports it), then
+ * A(i, j) <- s_i A(i, j) s_j on the lower tiles, and the scaling of
+ * transposed right-hand sides (element (c, r) at c + r nrp) by s_r. */
+__global__ void diag_rsqrt_kernel(const double *A, int NT, int ldt, double *sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NT * ldt) return;
+    const int t = i / ldt, r = i % ldt;
+    const double d = A[((size_t)t * NT + t) * ldt * ldt + r + (size_t)r * ldt];
+    sc[i] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+}
+
+__global__ void scale_lower_kernel(double *A, int NT, int ldt, const double *sc)
+{
+    const size_t ts = (size_t)ldt * ldt;
+    const int pair = blockIdx.y;
+    int i = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= pair) ++i;
+    while (i * (i + 1) / 2 > pair) --i;
+    const int m = i, n = pair - i * (i + 1) / 2;
+    double *T = A + ((size_t)n * NT + m) * ts;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < ts; e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % ldt), c = (int)(e / ldt);
+        T[e] *= sc[m * ldt + r] * sc[n * ldt + c];
+    }
+}
+
+__global__ void scale_bt_kernel(double *BT, int nrp, int rows, const double *sc)
+{
+    const size_t total = (size_t)nrp * rows;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+        BT[e] *= sc[e / nrp];
+}
+
+void diag_scaling(double *A, int NT, int ldt, double *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(diag_rsqrt_kernel, dim3((NT * ldt + 255) / 256), dim3(256), 0, s, A, NT, ldt, sc);
+    hipLaunchKernelGGL(scale_lower_kernel, dim3(std::min(64, (ldt * ldt + 255) / 256), NT * (NT + 1) / 2), dim3(256),
+                       0, s, A, NT, ldt, sc);
+}
+
+void scale_bt(double *BT, int nrp, int rows, const double *sc, hipStream_t s)
+{
+    const size_t total = (size_t)nrp * rows;
+    if (!total) return;
+    hipLaunchKernelGGL(scale_bt_kernel, dim3((unsigned)std::min<size_t>((total + 255) / 256, 8192)), dim3(256), 0, s,
+                       BT, nrp, rows, sc);
+}
+
+/* ------------------------------------------------------------------ */
+/* Squared Frobenius norms of the lower tiles (m >= n) of the symmetric
+ * matrix whose lower triangle the mirror holds: off-diagonal tiles whole,
+ * diagonal tiles as the symmetric tile (strictly lower entries twice, the
+ * upper part of the stored tile not read).  One block per tile, ss[m + n NT]. */
+__global__ __launch_bounds__(256) void tile_sumsq_lower_kernel(const double *A, int NT, int ldt, double *ss)
+{
+    __shared__ double red[256];
+    const int pair = blockIdx.x;
+    int i = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= pair) ++i;
+    while (i * (i + 1) / 2 > pair) --i;
+    const int m = i, n = pair - i * (i + 1) / 2;
+    const size_t ts = (size_t)ldt * ldt;
+    const double *T = A + ((size_t)n * NT + m) * ts;
+    double acc = 0.0;
+    for (size_t e = threadIdx.x; e < ts; e += 256) {
+        const int r = (int)(e % ldt), c = (int)(e / ldt);
+        const double v = T[e];
+        if (m != n) acc += v * v;
+        else if (r > c) acc += 2.0 * v * v;
+        else if (r == c) acc += v * v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ss[m + (size_t)n * NT] = red[0];
+}
+
+void tile_sumsq_lower(const double *A, int NT, int ldt, double *ss, hipStream_t s)
+{
+    if (NT <= 0) return;
+    hipLaunchKernelGGL(tile_sumsq_lower_kernel, dim3(NT * (NT + 1) / 2), dim3(256), 0, s, A, NT, ldt, ss);
+}
+
 void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int lmt, int ldt, int n0, int n1, int NT,
-                        const int *info, hipStream_t s)
+                        const int *info, hipStream_t s, const unsigned char *pmap)
 {
     if (n0 >= NT || n1 <= n0) return;
     int ncol = 0;
@@ -457,7 +756,10 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
     if (total <= 0) return;
     if (mode == 0)
         hipLaunchKernelGGL((update_f32_kernel<128, 128, 64, 32>), dim3((unsigned)total), dim3(512), 0, s, A,
-                           (const float *)P, (const float *)P2, lmt, ldt, n0, ncol, (int)total, info);
+                           (const float *)P, (const float *)P2, lmt, ldt, n0, ncol, (int)total, info, pmap);
+    else if (mode == 3)
+        hipLaunchKernelGGL(update_fp8_kernel, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned char *)P,
+                           (const unsigned char *)P2, NT, lmt, ldt, n0, ncol, (int)total, info);
     else {
         static const int pipelined = [] {
             const char *e = getenv("CHAMAMD_LP_PIPE");
@@ -466,7 +768,7 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
         auto *kern = pipelined ? (mode == 1 ? update_bf16p_kernel<1> : update_bf16p_kernel<3>)
                                : (mode == 1 ? update_bf16_kernel<1> : update_bf16_kernel<3>);
         hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
-                           (const unsigned short *)P2, NT, lmt, ldt, n0, ncol, (int)total, info);
+                           (const unsigned short *)P2, NT, lmt, ldt, n0, ncol, (int)total, info, pmap);
     }
 }
 

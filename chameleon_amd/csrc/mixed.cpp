@@ -19,8 +19,11 @@
  *
  * Reduced precision here means reduced-precision *products*: the working
  * copy of A stays in FP64 storage, its panels are factored in FP64, and the
- * trailing updates multiply FP32, BF16 or split-BF16 operands on the
- * matching MFMA units with FP32 accumulation (kernels/mixed.hip).
+ * trailing updates multiply FP32, BF16, split-BF16 or FP8 (E4M3, a
+ * power-of-two scale per tile) operands on the matching MFMA units with
+ * FP32 accumulation (kernels/mixed.hip).  By default the working copy is
+ * first scaled symmetrically to a unit diagonal (S A S, S = diag(A)^-1/2),
+ * and solves with its factor are scaled back.
  */
 #include "internal.h"
 #include "chameleon_amd.h"
@@ -70,6 +73,48 @@ std::vector<double> colmax_host(const double *V, int nrp, int nvalid, int rows, 
     CHX_HIP(hipMemcpyAsync(h.data(), dwork, nvalid * sizeof(double), hipMemcpyDeviceToHost, c->s_main));
     CHX_HIP(hipStreamSynchronize(c->s_main));
     return h;
+}
+
+/* Per-tile precision of the trailing products, Chameleon's rule for
+ * demoting tiles (compute/pzhered.c:176-288, runtime/starpu/codelets/
+ * codelet_zgered.c:62-163): with gnorm = ||A||_F of the symmetric matrix
+ * and threshold = u_high gnorm / NT (u_high: the requested accuracy,
+ * 1e-12 unless set, compute/zhered.c:170-176), tile (m,n) takes the
+ * lowest precision whose unit roundoff u_low still has
+ * ||A_mn||_F < threshold / u_low: BF16 (u_low = 2^-8, the half-width type
+ * with MFMA on CDNA4, where the reference's CUDA path uses FP16 with
+ * u_low = 1e-4), FP32 (u_low = 2^-24, slamch), else FP64.  Diagonal tiles
+ * use threshold_diag = threshold while u_high < 2^-24, else 2^-24 gnorm/NT
+ * (pzhered.c:249).  The GEMM/SYRK of an output tile then runs in that
+ * tile's precision, as the reference's dtype-dispatched gemm does
+ * (runtime/starpu/codelets/codelet_gemm.c:35-120: the C tile's type). */
+void tile_precision_map(const double *Aw, int NT, int ldt, double *tss, unsigned char *dmap, hipStream_t s)
+{
+    Context *c = ctx();
+    k::tile_sumsq_lower(Aw, NT, ldt, tss, s);
+    std::vector<double> ss((size_t)NT * NT, 0.0);
+    CHX_HIP(hipMemcpyAsync(ss.data(), tss, ss.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipStreamSynchronize(s));
+    double g2 = 0.0;
+    for (int n = 0; n < NT; ++n)
+        for (int m = n; m < NT; ++m) g2 += (m == n ? 1.0 : 2.0) * ss[m + (size_t)n * NT];
+    const double gnorm = std::sqrt(g2);
+    const double u_high = c->tile_accuracy, u32 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -8);
+    const double threshold = u_high * gnorm / NT;
+    const double threshold_diag = u_high < u32 ? threshold : u32 * gnorm / NT;
+    std::vector<unsigned char> map((size_t)NT * NT, 0);
+    long long cnt[3] = {0, 0, 0};
+    for (int n = 0; n < NT; ++n)
+        for (int m = n; m < NT; ++m) {
+            const double t = (m == n) ? threshold_diag : threshold;
+            const double ln = std::sqrt(ss[m + (size_t)n * NT]);
+            const unsigned char pr = ln < t / u16 ? 2 : (ln < t / u32 ? 1 : 0);
+            map[m + (size_t)n * NT] = pr;
+            ++cnt[pr];
+        }
+    for (int i = 0; i < 3; ++i) c->tile_counts[i] = cnt[i];
+    CHX_HIP(hipMemcpyAsync(dmap, map.data(), map.size(), hipMemcpyHostToDevice, s));
+    CHX_HIP(hipStreamSynchronize(s));
 }
 
 int check_descs(const char *fn, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X)
@@ -160,9 +205,18 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         const size_t pbytes = (size_t)NT * ts * 4;
         const size_t vb = (size_t)nrp * rows;
         const size_t pv = std::max(k::residual_scratch(NT, ldt, nrp), k::copy_lower_norm_scratch(NT, ldt));
-        char *w = (char *)workspace2(abytes + 4 * pbytes + (3 * vb + nrp + pv) * sizeof(double));
+        /* per-tile precision: a second set of panel buffers (BF16) and the map */
+        const bool adaptive = prec == CHAMAMD_PREC_ADAPTIVE;
+        const int npanel = adaptive ? 8 : 4;
+        const size_t mapbytes = adaptive ? (((size_t)NT * NT + 255) & ~(size_t)255) : 0;
+        char *w = (char *)workspace2(abytes + npanel * pbytes + mapbytes +
+                                     (3 * vb + nrp + pv + rows + (size_t)NT * NT) * sizeof(double));
         double *Aw = (double *)w;
-        double *BT = (double *)(w + abytes + 4 * pbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb, *part = nw + nrp;
+        unsigned char *dmap = (unsigned char *)(w + abytes + npanel * pbytes);
+        double *BT = (double *)(w + abytes + npanel * pbytes + mapbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb,
+               *part = nw + nrp;
+        double *sc = part + pv;       /* diagonal scaling of the working copy */
+        double *tss = sc + rows;      /* tile norms of the per-tile precision choice */
         /* the factorisation reads and writes the lower tiles only: copy
          * those and form ||A||_inf in the same pass (row sums into RT) */
         double anrm = 0.0;
@@ -176,6 +230,11 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
             CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
         }
         const double cte = anrm * eps * std::sqrt((double)N);
+        /* the working copy becomes S A S, S = diag(A)^-1/2; a solve with
+         * it is X = S (S A S)^-1 S B */
+        const bool scaled = c->mixed_scaling != 0;
+        if (scaled) k::diag_scaling(Aw, NT, ldt, sc, s);
+        if (adaptive) tile_precision_map(Aw, NT, ldt, tss, dmap, s);
 
         k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
         CHX_HIP(hipMemcpyAsync(XT, BT, vb * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -188,23 +247,30 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         p.d_info = st->d_info;
         p.lp_mode = prec;
         for (int i = 0; i < 4; ++i) p.P[i] = w + abytes + i * pbytes;
+        if (adaptive) {
+            for (int i = 0; i < 4; ++i) p.P2[i] = w + abytes + (4 + i) * pbytes;
+            p.pmap = dmap;
+        }
         /* X0 = A_lp^-1 B: with the fused panel the forward solve runs beside
          * the factorisation, sized for the rate of this precision's bulk
          * update (measured on MI355X, FP64-equivalent flop/s) */
         const bool overlap = potrf_solve_eligible(ldt);
+        if (scaled) k::scale_bt(XT, nrp, rows, sc, s);
         if (overlap) {
-            static const double rate[3] = {98e12, 377e12, 205e12};     /* FP32, BF16, BF16x3 */
+            /* FP32, BF16, BF16x3, FP8, per-tile (sized for its FP64 tiles) */
+            static const double rate[5] = {98e12, 377e12, 205e12, 400e12, 60e12};
             potrf_solve_run(p, XT, nrp, X->n, rate[prec]);
         } else {
             potrf_run(p);
         }
         st->pending_info = true;
         const int info = sequence_wait_internal(seq);
-        if (info != 0) {
+        if (info != 0 || *st->h_info == LP_RANGE) {
             it_count = -3;
         } else {
             SolvePlan sp = solve_plan(Aw, NT, ldt, nrp, st->d_info);
             if (!overlap) solve_bt(sp, Aw, NT, ldt, XT, nrp, st->d_info);
+            if (scaled) k::scale_bt(XT, nrp, rows, sc, s);
             for (int it = 0; it <= ITERMAX; ++it) {
                 k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, part, s);
                 std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
@@ -226,7 +292,9 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
                     break;
                 }
                 if (it == ITERMAX) break;
+                if (scaled) k::scale_bt(RT, nrp, rows, sc, s);
                 solve_bt(sp, Aw, NT, ldt, RT, nrp, st->d_info);
+                if (scaled) k::scale_bt(RT, nrp, rows, sc, s);
                 k::axpy(XT, RT, vb, s);
             }
             /* a strip-solve wait that ran out leaves its code in the info
@@ -277,7 +345,7 @@ static int mixed_checked(const char *fn, cham_uplo_t uplo, CHAM_desc_t *A, CHAM_
         error(fn, "illegal value of uplo");
         return -1;
     }
-    if (prec < CHAMAMD_PREC_FP32 || prec > CHAMAMD_PREC_BF16X3) {
+    if (prec < CHAMAMD_PREC_FP32 || prec > CHAMAMD_PREC_ADAPTIVE) {
         error(fn, "illegal value of prec");
         return -5;
     }
@@ -316,6 +384,38 @@ int CHAMAMD_dposv_mixed_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t
     request->status = CHAMELEON_SUCCESS;
     int rc = mixed_checked(fn, uplo, A, B, X, prec, iter, berr);
     return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
+}
+
+int CHAMAMD_Set_Tile_Accuracy(double u_high)
+{
+    if (!ctx()) {
+        error("CHAMAMD_Set_Tile_Accuracy", "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    if (!(u_high > 0.0)) {
+        error("CHAMAMD_Set_Tile_Accuracy", "the accuracy must be > 0");
+        return CHAMELEON_ERR_ILLEGAL_VALUE;
+    }
+    ctx()->tile_accuracy = u_high;
+    return CHAMELEON_SUCCESS;
+}
+
+int CHAMAMD_Mixed_Tile_Counts(long long *counts)
+{
+    if (!ctx()) return CHAMELEON_ERR_NOT_INITIALIZED;
+    if (!counts) return CHAMELEON_ERR_ILLEGAL_VALUE;
+    for (int i = 0; i < 3; ++i) counts[i] = ctx()->tile_counts[i];
+    return CHAMELEON_SUCCESS;
+}
+
+int CHAMAMD_Set_Mixed_Scaling(int on)
+{
+    if (!ctx()) {
+        error("CHAMAMD_Set_Mixed_Scaling", "CHAMELEON not initialized");
+        return CHAMELEON_ERR_NOT_INITIALIZED;
+    }
+    ctx()->mixed_scaling = on ? 1 : 0;
+    return CHAMELEON_SUCCESS;
 }
 
 int CHAMAMD_Set_Refinement_Target(double berr)

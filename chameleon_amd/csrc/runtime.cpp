@@ -343,6 +343,12 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     c->panel = env_int("CHAMAMD_PANEL", 1);
     c->pairs = env_int("CHAMAMD_PAIRS", 1);
     c->fwd_tail = env_int("CHAMAMD_FWD_TAIL", -1);
+    {
+        /* requested accuracy of the per-tile precision choice, as the
+         * reference's zhered reads it (compute/zhered.c:170-176) */
+        const char *acc = getenv("CHAMELEON_GERED_ACC");
+        if (acc && *acc && strtod(acc, nullptr) > 0.0) c->tile_accuracy = strtod(acc, nullptr);
+    }
     CHX_HIP(hipSetDevice(c->device));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_update, hipStreamNonBlocking));

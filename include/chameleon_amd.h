@@ -50,8 +50,36 @@ int CHAMAMD_Set_Forward_Tail(int tiles);
 enum {
     CHAMAMD_PREC_FP32   = 0,   /* FP32 operands, v_mfma_f32_16x16x4_f32                      */
     CHAMAMD_PREC_BF16   = 1,   /* BF16 operands, v_mfma_f32_16x16x32_bf16                    */
-    CHAMAMD_PREC_BF16X3 = 2    /* operands split hi+lo in BF16, 3 products (~2^-16 accuracy) */
+    CHAMAMD_PREC_BF16X3 = 2,   /* operands split hi+lo in BF16, 3 products (~2^-16 accuracy) */
+    CHAMAMD_PREC_FP8    = 3,   /* OCP E4M3 operands, one power-of-two scale per tile,
+                                  v_mfma_scale_f32_16x16x128_f8f6f4 (2^-4 accuracy: refinement
+                                  converges for condition numbers up to ~10)                  */
+    CHAMAMD_PREC_ADAPTIVE = 4  /* per-tile precision by Chameleon's tile-norm rule
+                                  (compute/pzhered.c:176-288): each trailing tile is updated
+                                  in FP64, FP32 or BF16 as its norm allows for the accuracy
+                                  of CHAMAMD_Set_Tile_Accuracy                                */
 };
+
+/* Requested accuracy u_high of the per-tile precision choice
+ * (CHAMAMD_PREC_ADAPTIVE): tile (m,n) of the working matrix is updated in
+ * BF16 when ||A_mn||_F < u_high ||A||_F / (NT 2^-8), else in FP32 when
+ * ||A_mn||_F < u_high ||A||_F / (NT 2^-24), else in FP64 -- the rule of
+ * the reference's zhered/zgered (default 1e-12, or the CHAMELEON_GERED_ACC
+ * environment variable as compute/zhered.c:170-176 reads it).  Returns
+ * CHAMELEON_SUCCESS, CHAMELEON_ERR_ILLEGAL_VALUE (u_high <= 0) or
+ * CHAMELEON_ERR_NOT_INITIALIZED. */
+int CHAMAMD_Set_Tile_Accuracy(double u_high);
+/* Lower tiles (m >= n) given FP64, FP32 and BF16 products by the last
+ * CHAMAMD_PREC_ADAPTIVE solve, in counts[0..2]. */
+int CHAMAMD_Mixed_Tile_Counts(long long *counts);
+
+/* Symmetric diagonal scaling of the working copy before demotion:
+ * D^-1/2 A D^-1/2 (D = diag(A)) is factored in reduced precision and the
+ * solves are scaled back, so the demoted operands span a range set by the
+ * matrix's structure rather than its diagonal's magnitudes; residuals stay
+ * on the original A in FP64.  on = 1 (the default) or 0.  Returns
+ * CHAMELEON_SUCCESS or CHAMELEON_ERR_NOT_INITIALIZED. */
+int CHAMAMD_Set_Mixed_Scaling(int on);
 
 /* Mixed-precision Cholesky solve X = A^-1 B with FP64 iterative refinement
  * and FP64 fallback (LAPACK dsposv semantics, see chameleon/chameleon_ds.h).

@@ -280,3 +280,32 @@ def latms(kappa, n, seed, dist=203):
     A = (Q * D) @ Q.T
     A = 0.5 * (A + A.T)
     return np.asfortranarray(A), D
+
+
+# ------------------------------------------------- per-tile precision rule
+def tile_precision_map(A, nb, u_high=1e-12):
+    """Precision of every lower tile (m >= n) of the symmetric matrix whose
+    lower triangle is A, by the rule of compute/pzhered.c:176-288 and
+    runtime/starpu/codelets/codelet_zgered.c:62-163: gnorm = ||A||_F,
+    threshold = u_high gnorm / NT, threshold_diag = threshold while
+    u_high < 2^-24 else 2^-24 gnorm / NT (pzhered.c:249); a tile goes to the
+    half-width type if ||A_mn||_F < t / u16 (here BF16, u16 = 2^-8), else to
+    FP32 if ||A_mn||_F < t / 2^-24, else stays FP64.  Returns a dict
+    (m, n) -> 0 / 1 / 2 (FP64 / FP32 / BF16)."""
+    n = A.shape[0]
+    S = np.tril(A) + np.tril(A, -1).T
+    nt = -(-n // nb)
+    ss = {}
+    for j in range(nt):
+        for i in range(j, nt):
+            ss[(i, j)] = float(np.sum(S[i * nb:(i + 1) * nb, j * nb:(j + 1) * nb] ** 2))
+    gnorm = np.sqrt(sum((1.0 if i == j else 2.0) * v for (i, j), v in ss.items()))
+    u32, u16 = 2.0 ** -24, 2.0 ** -8
+    thr = u_high * gnorm / nt
+    thr_diag = thr if u_high < u32 else u32 * gnorm / nt
+    out = {}
+    for (i, j), v in ss.items():
+        t = thr_diag if i == j else thr
+        ln = np.sqrt(v)
+        out[(i, j)] = 2 if ln < t / u16 else (1 if ln < t / u32 else 0)
+    return out

@@ -116,10 +116,13 @@ static inline int tdim(int k, int nb, int n) { return (k + 1) * nb > n ? n - k *
 void orc_dplgsy(double bump, int uplo, int N, double *A, int lda, int nb, uint64_t seed)
 {
     int nt = (N + nb - 1) / nb;
-    for (int m = 0; m < nt; m++) {
-        int mm = tdim(m, nb, N), m0 = m * nb;
-        for (int n = 0; n < nt; n++) {
-            int nn = tdim(n, nb, N), n0 = n * nb;
+    /* tiles are independent (every element's stream position is fixed):
+     * tile columns are generated in parallel */
+    #pragma omp parallel for schedule(dynamic)
+    for (int n = 0; n < nt; n++) {
+        int nn = tdim(n, nb, N), n0 = n * nb;
+        for (int m = 0; m < nt; m++) {
+            int mm = tdim(m, nb, N), m0 = m * nb;
             if (uplo == Lower && n0 >= m0 + mm) continue;
             if (uplo == Upper && m0 >= n0 + nn) continue;
             orc_dplgsy_tile(bump, mm, nn, A + (size_t)n0 * lda + m0, lda, N, m0, n0, seed);
@@ -130,8 +133,9 @@ void orc_dplgsy(double bump, int uplo, int N, double *A, int lda, int nb, uint64
 void orc_dplrnt(int M, int N, double *A, int lda, int nb, uint64_t seed)
 {
     int mt = (M + nb - 1) / nb, nt = (N + nb - 1) / nb;
-    for (int m = 0; m < mt; m++)
-        for (int n = 0; n < nt; n++)
+    #pragma omp parallel for schedule(dynamic)
+    for (int n = 0; n < nt; n++)
+        for (int m = 0; m < mt; m++)
             orc_dplrnt_tile(tdim(m, nb, M), tdim(n, nb, N),
                             A + (size_t)n * nb * lda + (size_t)m * nb, lda, M, m * nb, n * nb, seed);
 }

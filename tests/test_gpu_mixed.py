@@ -19,7 +19,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 L_, U_, UL_ = 122, 121, 123
-FP32, BF16, BF16X3 = 0, 1, 2
+FP32, BF16, BF16X3, FP8 = 0, 1, 2, 3
 
 
 @pytest.fixture(scope="module")
@@ -66,7 +66,7 @@ def mixed(ch, uplo, A, B, prec, nb):
     return rc, X, it.value, be.value
 
 
-@pytest.mark.parametrize("prec", [FP32, BF16X3, BF16])
+@pytest.mark.parametrize("prec", [FP32, BF16X3, BF16, FP8])
 @pytest.mark.parametrize("uplo", [L_, U_])
 @pytest.mark.parametrize("n,nb,nrhs", [(1024, 256, 1), (1536, 256, 16), (2048, 512, 64), (1000, 128, 3),
                                        (1150, 128, 7), (2500, 256, 32)])
@@ -87,7 +87,7 @@ def test_mixed_well_conditioned(ch, prec, uplo, n, nb, nrhs):
 
 
 @pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
-@pytest.mark.parametrize("prec", [FP32, BF16X3, BF16])
+@pytest.mark.parametrize("prec", [FP32, BF16X3, BF16, FP8])
 def test_mixed_conditioned(ch, kappa, prec):
     n, nb, nrhs = 1024, 256, 4
     say("mixed conditioned", kappa, prec)
@@ -234,3 +234,93 @@ def test_backward_error_of_fp64_solution(ch):
               for j in range(nrhs))
     # both are rounding-level quantities summed in different orders
     assert 0.0 < be.value < 1.0 and 0.25 <= be.value / ref <= 4.0
+
+
+@pytest.mark.parametrize("prec", [FP32, BF16, FP8])
+def test_mixed_diagonal_scaling(ch, prec):
+    """A = D^1/2 A0 D^1/2, A0 the bumped dplgsy matrix (condition number
+    ~1) and D spanning 1e-6 .. 1e6: the symmetric diagonal scaling hands the
+    low-precision factorisation A0 itself, so refinement converges in every
+    precision, FP8 included; without the scaling the answer is still the
+    FP64 one (converged or by fallback)."""
+    n, nb, nrhs = 1536, 256, 4
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    d = np.logspace(-6, 6, n)[np.random.default_rng(2).permutation(n)]
+    As = np.asfortranarray(np.sqrt(d)[:, None] * A0 * np.sqrt(d)[None, :])
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    Xr = np.linalg.solve(As, B)
+    try:
+        for on in (1, 0):
+            say("mixed scaling", prec, on)
+            assert ch.lib.CHAMAMD_Set_Mixed_Scaling(on) == 0
+            A = As.copy(order="F")
+            rc, X, it, be = mixed(ch, L_, A, B, prec, nb)
+            assert rc == 0
+            if on:
+                assert 0 <= it <= 30, it
+            assert oracle.rel_frobenius(X, Xr) <= 1e-12
+            assert oracle.check_solve(As, X, B)[0]
+    finally:
+        ch.lib.CHAMAMD_Set_Mixed_Scaling(1)
+
+
+def test_mixed_fp8_nonfinite_falls_back(ch):
+    """A NaN below the diagonal of the first panel reaches the FP8 demotion,
+    whose range check stops the low-precision factorisation (ITER = -3);
+    the FP64 fallback then reports LAPACK's info for the same matrix."""
+    n, nb = 512, 128
+    say("mixed fp8 nan")
+    A = oracle.dplgsy(float(n), UL_, n, 51)
+    A[300, 5] = A[5, 300] = np.nan
+    _, info_ref = __import__("scipy.linalg.lapack", fromlist=["dpotrf"]).dpotrf(A.copy(), lower=True)
+    B = np.ones((n, 1), order="F")
+    rc, X, it, be = mixed(ch, L_, A.copy(order="F"), B, FP8, nb)
+    assert it == -3
+    assert rc == info_ref > 0
+
+
+ADAPTIVE = 4
+
+
+@pytest.mark.parametrize("u_high", [1e-12, 1e-6, 1e-3])
+@pytest.mark.parametrize("n,nb", [(2048, 256), (1150, 128)])
+def test_mixed_per_tile_precision(ch, u_high, n, nb):
+    """Per-tile precision (CHAMAMD_PREC_ADAPTIVE): the tiles given FP64,
+    FP32 and BF16 products are those of the oracle's restatement of
+    Chameleon's zhered rule on the scaled working matrix S A S, and the
+    refined answer is the FP64 one whatever the mix."""
+    say("mixed per-tile", u_high, n, nb)
+    A0 = oracle.dplgsy(float(n), UL_, n, 51)
+    B = oracle.dplrnt(n, 3, 5673, nb=nb)
+    d = 1.0 / np.sqrt(np.diag(A0))
+    want = oracle.tile_precision_map(d[:, None] * A0 * d[None, :], nb, u_high)
+    counts = [sum(1 for v in want.values() if v == p) for p in (0, 1, 2)]
+    try:
+        assert ch.lib.CHAMAMD_Set_Tile_Accuracy(u_high) == 0
+        A = oracle.dplgsy(float(n), L_, n, 51, nb=nb)
+        rc, X, it, be = mixed(ch, L_, A, B, ADAPTIVE, nb)
+        got = (ctypes.c_longlong * 3)()
+        assert ch.lib.CHAMAMD_Mixed_Tile_Counts(got) == 0
+    finally:
+        ch.lib.CHAMAMD_Set_Tile_Accuracy(1e-12)
+    assert rc == 0 and 0 <= it <= 30, it
+    assert list(got) == counts, (list(got), counts)
+    assert oracle.rel_frobenius(X, np.linalg.solve(A0, B)) <= 1e-13
+    assert oracle.check_solve(A0, X, B)[0]
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+def test_mixed_per_tile_precision_conditioned(ch, kappa):
+    n, nb = 1024, 256
+    say("mixed per-tile conditioned", kappa)
+    A = conditioned_spd(n, kappa, 7)
+    A0 = A.copy(order="F")
+    B = np.asfortranarray(np.random.default_rng(3).uniform(-0.5, 0.5, (n, 2)))
+    try:
+        assert ch.lib.CHAMAMD_Set_Tile_Accuracy(1e-5) == 0
+        rc, X, it, be = mixed(ch, L_, A, B, ADAPTIVE, nb)
+    finally:
+        ch.lib.CHAMAMD_Set_Tile_Accuracy(1e-12)
+    assert rc == 0
+    assert oracle.rel_frobenius(X, np.linalg.solve(A0, B)) <= kappa * 1e-13
+    assert oracle.check_solve(A0, X, B)[0]

@@ -75,3 +75,31 @@ def test_latms_symmetric_dist_uses_shifted_dplrnt():
     Q, _ = np.linalg.qr(G)
     A, D = oracle.latms(10.0, n, seed, 202)
     assert np.allclose(Q.T @ A @ Q, np.diag(D), atol=1e-14)
+
+
+def test_tile_precision_rule_closed_form():
+    """Block matrix with known tile norms: diagonal tiles 4 I (norm 8 for
+    nb = 4), off-diagonal tiles c J (all entries c, norm 4 c).  gnorm^2 =
+    NT * 64 + NT (NT - 1) * 16 c^2; the choice follows from the thresholds."""
+    nb, nt = 4, 3
+    for c, u_high in [(1e-3, 1e-12), (1e-3, 1e-6), (1e-8, 1e-6), (1e-9, 1e-3)]:
+        A = np.full((nb * nt, nb * nt), c)
+        for t in range(nt):
+            A[t * nb:(t + 1) * nb, t * nb:(t + 1) * nb] = 4.0 * np.eye(nb)
+        got = oracle.tile_precision_map(A, nb, u_high)
+        g = np.sqrt(nt * 64.0 + nt * (nt - 1) * 16.0 * c * c)
+        thr = u_high * g / nt
+        thr_d = thr if u_high < 2.0 ** -24 else 2.0 ** -24 * g / nt
+        want_off = 2 if 4 * c < thr / 2.0 ** -8 else (1 if 4 * c < thr / 2.0 ** -24 else 0)
+        want_diag = 2 if 8 < thr_d / 2.0 ** -8 else (1 if 8 < thr_d / 2.0 ** -24 else 0)
+        for (i, j), v in got.items():
+            assert v == (want_diag if i == j else want_off), (c, u_high, i, j, v)
+    # the cases above reach all three precisions
+    def block(c):
+        A = np.full((12, 12), c)
+        for t in range(3):
+            A[t * 4:(t + 1) * 4, t * 4:(t + 1) * 4] = 4.0 * np.eye(4)
+        return A
+    assert set(oracle.tile_precision_map(block(1e-9), 4, 1e-3).values()) == {0, 2}
+    assert set(oracle.tile_precision_map(block(1e-3), 4, 1e-6).values()) == {0, 1}
+    assert set(oracle.tile_precision_map(block(1e-3), 4, 1e-12).values()) == {0}
