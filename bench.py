@@ -489,7 +489,7 @@ def main():
     ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--kappas", default="1e2,1e4,1e6", help="condition numbers of the mixed-precision leg")
     ap.add_argument("--mixed-nb", type=int, default=0, help="tile size of the mixed-precision leg (0: the FP64 one)")
-    ap.add_argument("--mixed-precs", default="fp32,bf16x3,bf16", help="products of the mixed-precision leg")
+    ap.add_argument("--mixed-precs", default="fp32,bf16x3,bf16,fp8", help="products of the mixed-precision leg")
     ap.add_argument("--mixed-nrhs", default="1,64", help="right-hand-side counts of the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve", type=float, default=3.0, metavar="SECONDS",

@@ -93,10 +93,10 @@ double host_norm(CHAM_desc_t *A, cham_normtype_t norm, int uplo)
     double r = 0.0;
     switch (norm) {
     case ChamOneNorm:
-        for (int j = 0; j < A->n; ++j) r = std::fmax(r, h[j]);
+        for (int j = 0; j < A->n; ++j) r = nan_max(r, h[j]);
         break;
     case ChamInfNorm:
-        for (int i = 0; i < A->m; ++i) r = std::fmax(r, h[A->n + i]);
+        for (int i = 0; i < A->m; ++i) r = nan_max(r, h[A->n + i]);
         break;
     case ChamMaxNorm:
         r = h[A->n + A->m];

@@ -38,26 +38,17 @@ using namespace chx;
 
 namespace chx {
 
-static hipEvent_t event(size_t i)
-{
-    Context *c = ctx();
-    while (c->events.size() <= i) {
-        hipEvent_t e;
-        CHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        c->events.push_back(e);
-    }
-    return c->events[i];
-}
-
 void potrf_run(const PotrfPlan &p)
 {
     Context *c = ctx();
-    hipStream_t sm = c->s_main, su = c->s_update;
+    Lane *L = p.lane ? p.lane : main_lane();
+    hipStream_t sm = L->sm, su = L->su;
+    auto event = [L](size_t i) { return lane_event(L, i); };
     const int NT = p.NT, ldt = p.ldt;
     const size_t ts = (size_t)ldt * ldt;
     /* explicit inverses of the 64 x 64 diagonal blocks, per diagonal tile */
     const size_t dper = (size_t)((ldt + 63) / 64) * 64 * 64;
-    double *dinv = (double *)workspace((size_t)NT * dper * sizeof(double));
+    double *dinv = p.dinv ? p.dinv : (double *)workspace((size_t)NT * dper * sizeof(double));
     auto tile = [&](int m, int n) { return p.A + ((size_t)n * NT + m) * ts; };
     /* algorithmic flops (include/chameleon/flops.h) of each launch, for
      * the kernel statistics */
@@ -74,7 +65,7 @@ void potrf_run(const PotrfPlan &p)
     };
     /* paired bulk updates: the trailing matrix right of the two lookahead
      * columns takes the panels of steps k and k+1 in one pass */
-    const bool pairs = c->pairs && c->lookahead > 0 && NT >= 4 && (p.lp_mode < 0 || p.P[3] != nullptr);
+    const bool pairs = c->pairs && c->lookahead > 0 && su && NT >= 4 && (p.lp_mode < 0 || p.P[3] != nullptr);
     auto pbuf = [&](int k) { return pairs ? p.P[k % 4] : p.P[k & 1]; };
     auto pbuf2 = [&](int k) { return pairs ? p.P2[k % 4] : p.P2[k & 1]; };
     auto update = [&](int k, int n0, int n1, hipStream_t s, int kp = 1) {
@@ -82,7 +73,7 @@ void potrf_run(const PotrfPlan &p)
         if (n0 >= NT || n1 <= n0) return;
         /* with lookahead, the critical stream only updates the lookahead
          * columns; they are timed apart from the bulk update */
-        const int cls = (s == sm && c->lookahead > 0) ? CHAMAMD_KERNEL_COLUPD : CHAMAMD_KERNEL_UPDATE;
+        const int cls = (s == sm && c->lookahead > 0 && su) ? CHAMAMD_KERNEL_COLUPD : CHAMAMD_KERNEL_UPDATE;
         timed(cls, kp * update_flops(k, n0, n1), s, [&] {
             if (p.lp_mode < 0) {
                 k::trailing_update(p.A, NT, ldt, k, n0, n1, NT, p.d_info, s, kp);
@@ -137,8 +128,8 @@ void potrf_run(const PotrfPlan &p)
             for (int m = k + 1; m < NT; ++m) f += tsz(m) * nbd * nbd;
             const int ND = ldt / 64;
             int epoch = 0;
-            int *flags = strip_flags(ND * ND, 1, &epoch);
-            unsigned *ctr = panel_counters();
+            int *flags = lane_flags(L, ND * ND, 1, &epoch);
+            unsigned *ctr = lane_counters(L);
             timed(CHAMAMD_KERNEL_PANEL, f, sm, [&] {
                 k::panel_factor(tile(k, k), tile(std::min(k + 1, NT - 1), k), (long long)ts, NT - k - 1, ldt, dk,
                                 flags, ctr, epoch, p.d_info, p.nb * k, nvalid, sm);
@@ -158,7 +149,7 @@ void potrf_run(const PotrfPlan &p)
             if (p.lp_mode >= 0) demote(k);
         }
     };
-    if (c->lookahead <= 0 || NT <= 2) {
+    if (c->lookahead <= 0 || !su || NT <= 2) {
         for (int k = 0; k < NT; ++k) {
             panel(k);
             update(k, k + 1, NT, sm);
@@ -281,6 +272,18 @@ void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nr
     solve_pair(p, A, NT, ldt, BT, nrp, d_info);
 }
 
+void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int ldt, double *BT, int nrp,
+                       const int *d_info)
+{
+    const int S = NT * ldt / 64, chunks = (nrp + 63) / 64;
+    int epoch = 0;
+    int *flags = lane_flags(L, 4 * S, 2 * chunks, &epoch);
+    const size_t dper = (size_t)(ldt / 64) * 64 * 64;
+    for (int dir = 0; dir < 2; ++dir)
+        k::solve_strips(A, dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
+                        d_info, L->sm);
+}
+
 /* B <- A^-1 B with the Lower factor in A: forward then backward solve on
  * the transposed right-hand sides. */
 static void solve_run(const SolvePlan &p, double *A, int NT, int ldt, double *B, int Bnt, int wdt, int *d_info,
@@ -400,6 +403,7 @@ struct FwdSolve {
     double *A, *dinv, *BT;
     size_t dper;
     int NT, ldt, nrp, nrhs, nb;
+    hipStream_t s;              /* the factorisation's critical stream */
     int next;                   /* first tile row not yet issued      */
     int last;                   /* last tile row issued beside potrf  */
     int *d_info;
@@ -408,7 +412,7 @@ struct FwdSolve {
 static void fwd_after_panels(void *arg, int k)
 {
     FwdSolve *f = (FwdSolve *)arg;
-    hipStream_t sm = ctx()->s_main;
+    hipStream_t sm = f->s;
     for (; f->next <= k && f->next <= f->last; ++f->next) {
         const int t = f->next;
         const double nb = f->nb, below = (double)(f->NT - t - 1) * nb;
@@ -448,12 +452,15 @@ bool potrf_solve_eligible(int ldt)
 void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate)
 {
     Context *c = ctx();
-    hipStream_t sm = c->s_main;
+    Lane *L = p.lane ? p.lane : main_lane();
+    hipStream_t sm = L->sm;
     const int NT = p.NT, ldt = p.ldt;
     FwdSolve f;
     f.dper = (size_t)(ldt / 64) * 64 * 64;
     /* where potrf_run puts the block inverses */
-    f.dinv = (double *)workspace((size_t)NT * f.dper * sizeof(double));
+    f.dinv = p.dinv ? p.dinv : (double *)workspace((size_t)NT * f.dper * sizeof(double));
+    p.dinv = f.dinv;
+    f.s = sm;
     f.A = p.A;
     f.BT = BT;
     f.NT = NT; f.ldt = ldt; f.nrp = nrp; f.nrhs = nrhs; f.nb = p.nb;
@@ -467,7 +474,7 @@ void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate)
     /* the rows not solved beside the factorisation, then backward */
     const int S = NT * ldt / 64, chunks = (nrp + 63) / 64, s0 = f.next * (ldt / 64);
     int epoch = 0;
-    int *flags = strip_flags(4 * S, 2 * chunks, &epoch);
+    int *flags = lane_flags(L, 4 * S, 2 * chunks, &epoch);
     const double n = (double)(NT - 1) * p.nb + p.nvalid_last, rows_left = n - (double)f.next * p.nb;
     timed(CHAMAMD_KERNEL_SOLVE, (rows_left * rows_left + n * n) * nrhs, sm, [&] {
         k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 0, flags, epoch,

@@ -34,6 +34,16 @@ void fatal_hip(hipError_t e, const char *what, const char *file, int line);
 #define PANEL_STALLED (-(1 << 30))
 /* info value of a low-precision demotion that met a non-finite entry */
 #define LP_RANGE (-(1 << 29))
+/* maximum that propagates NaN, as LAPACK's norms do (dlange/dlansy check
+ * every entry with disnan): a NaN operand makes the result NaN */
+__host__ __device__ inline double nan_max(double a, double b)
+{
+    return (a != a || b != b) ? a + b : (a > b ? a : b);
+}
+__host__ __device__ inline float nan_maxf(float a, float b)
+{
+    return (a != a || b != b) ? a + b : (a > b ? a : b);
+}
 #define CHX_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) ::chx::fatal_hip(_e, #call, __FILE__, __LINE__); } while (0)
 
 void latms_release();      /* frees the conditioned generator's kept factor */
@@ -53,6 +63,28 @@ struct Mirror {
     bool    host_valid = true;    /* host copy up to date                   */
     bool    diag_identity = false;/* padding of diagonal tiles is identity  */
 };
+
+/* Where a factorisation or a strip solve runs: the critical-path stream,
+ * an optional bulk-update stream (null: everything on the critical stream,
+ * as without lookahead), the events that order the two, the fused panel
+ * kernel's ticket counters and the strip solves' publication flags (tagged
+ * by launch epoch, so they never need clearing between launches).  The
+ * context owns one lane for the library's own calls; the solve server owns
+ * one per factorisation or solve it runs at the same time. */
+struct Lane {
+    hipStream_t sm = nullptr, su = nullptr;
+    std::vector<hipEvent_t> events;
+    int *flags = nullptr;
+    int flags_n = 0;
+    int epoch = 0;
+    unsigned *panel_ctr = nullptr;
+};
+Lane *main_lane();
+hipEvent_t lane_event(Lane *L, size_t i);
+/* at least n flags and the first of `launches` fresh epochs */
+int *lane_flags(Lane *L, int n, int launches, int *epoch);
+unsigned *lane_counters(Lane *L);
+void lane_release(Lane *L);         /* frees flags, counters and events (not the streams) */
 
 struct Context {
     int device = 0;
@@ -78,17 +110,12 @@ struct Context {
     size_t ws2_bytes = 0;
     void  *pinned = nullptr;
     size_t pinned_bytes = 0;
-    std::vector<hipEvent_t> events;
+    Lane lane;                        /* s_main / s_update and their events, flags, counters */
     /* kernel statistics (CHAMELEON_GENERATE_STATS) */
     bool stats = false;
     struct Pending { int cls; hipEvent_t a, b; double flops; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> timing_pool;
-    /* publication flags of the strip solves, tagged by launch epoch */
-    int *strip_flags = nullptr;
-    int strip_flags_n = 0;
-    int strip_epoch = 0;
-    unsigned *panel_ctr = nullptr;  /* ticket / finished counters of the fused panel kernel */
     int panel = 1;                  /* fused panel kernel (CHAMAMD_PANEL=0: POTRF + TRSM)   */
     int pairs = 1;                  /* paired bulk updates (CHAMAMD_PAIRS=0: one per step)  */
     void *latms_cache = nullptr;    /* orthogonal factor kept by the conditioned generator   */
@@ -166,8 +193,18 @@ struct PotrfPlan {
      * their diagonal blocks in the potrf workspace) */
     void (*after_panels)(void *arg, int k) = nullptr;
     void *after_arg = nullptr;
+    /* where it runs (null: the context's lane) and where the 64 x 64
+     * diagonal-block inverses go (null: the context workspace; the solves
+     * of the same factor read them there) */
+    Lane *lane = nullptr;
+    double *dinv = nullptr;
 };
 void potrf_run(const PotrfPlan &p);
+/* both strip solves of BT (nrp x NT*ldt, transposed right-hand sides) with
+ * the Lower factor A and its 64 x 64 diagonal-block inverses, on lane L;
+ * ldt a multiple of 64 */
+void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int ldt, double *BT, int nrp,
+                       const int *d_info);
 /* potrf_run with the forward solve of BT (transposed right-hand sides,
  * nrp x NT*ldt, nrhs of them in use) issued beside the factorisation, then
  * the forward strips it left and the backward solve: BT <- A^-1 BT.  The

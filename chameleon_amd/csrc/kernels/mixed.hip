@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void tile_absmax_kernel(const double *A, int N
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     float mx = 0.f;
     for (int jj = ty; jj < 32; jj += 8)
-        if (r0 + tx < ldt && c0 + jj < ldt) mx = fmaxf(mx, (float)fabs(T[(r0 + tx) + (size_t)(c0 + jj) * ldt]));
+        if (r0 + tx < ldt && c0 + jj < ldt) mx = nan_maxf(mx, (float)fabs(T[(r0 + tx) + (size_t)(c0 + jj) * ldt]));
     /* non-negative floats order like their bits; a NaN compares above inf */
     unsigned u = __float_as_uint(mx);
     if (mx != mx) u = 0x7fc00000u;
@@ -964,8 +964,8 @@ __global__ void colmax_kernel(const double *V, int nrp, int nrows, double *out)
     const int j = blockIdx.y;
     double mx = 0.0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nrows; p += gridDim.x * blockDim.x)
-        mx = fmax(mx, fabs(V[j + (size_t)p * nrp]));
-    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off));
+        mx = nan_max(mx, fabs(V[j + (size_t)p * nrp]));
+    for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off));
     if ((threadIdx.x & 63) == 0)
         atomicMax(reinterpret_cast<unsigned long long *>(&out[j]), (unsigned long long)__double_as_longlong(mx));
 }

@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void norm_kernel(const double *d, Geo g, int u
             const double v = fabs(tile[i + (size_t)j * g.ldt]);
             s += v;
             if (diagonal(i, j)) { sd += v; ss += v * v; } else ss += (sym ? 2.0 : 1.0) * v * v;
-            mx = fmax(mx, v);
+            mx = nan_max(mx, v);
         }
         /* tile row i is logical row r0 + i (no trans) or column c0 + i (trans) */
         if (!trans) {
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void norm_kernel(const double *d, Geo g, int u
     }
     /* wave reduction then one atomic per wave */
     for (int off = 32; off > 0; off >>= 1) {
-        mx = fmax(mx, __shfl_down(mx, off));
+        mx = nan_max(mx, __shfl_down(mx, off));
         ss += __shfl_down(ss, off);
     }
     if (lane == 0) {

@@ -59,7 +59,7 @@ double norm_inf_sym(CHAM_desc_t *A)
     CHX_HIP(hipMemcpyAsync(h.data(), work + A->n, A->m * sizeof(double), hipMemcpyDeviceToHost, c->s_main));
     CHX_HIP(hipStreamSynchronize(c->s_main));
     double r = 0.0;
-    for (double v : h) r = std::fmax(r, v);
+    for (double v : h) r = nan_max(r, v);
     return r;
 }
 
@@ -171,7 +171,7 @@ int backward_error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t
     for (int j = 0; j < X->n; ++j) {
         const int pj = (j / X->nb) * MX->wdt + (j % X->nb);   /* padded column index */
         const double d = anrm * xn[pj] * A->m * eps;
-        worst = std::fmax(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
+        worst = nan_max(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
     }
     *berr = worst;
     return CHAMELEON_SUCCESS;
@@ -224,7 +224,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
             std::vector<double> h(N);
             CHX_HIP(hipMemcpyAsync(h.data(), RT, N * sizeof(double), hipMemcpyDeviceToHost, s));
             CHX_HIP(hipStreamSynchronize(s));
-            for (double v : h) anrm = std::fmax(anrm, v);
+            for (double v : h) anrm = nan_max(anrm, v);
         } else {
             anrm = norm_inf_sym(A);
             CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
@@ -281,7 +281,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
                     const int pj = (j / X->nb) * MX->wdt + (j % X->nb);
                     if (!(rn[pj] <= xn[pj] * cte)) ok = false;
                     const double d = anrm * xn[pj] * N * eps;
-                    worst = std::fmax(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
+                    worst = nan_max(worst, d > 0 ? rn[pj] / d : (rn[pj] > 0 ? INFINITY : 0.0));
                 }
                 /* with a target, refinement goes on (converged or not by the
                  * test above) until every column's backward error is within it */

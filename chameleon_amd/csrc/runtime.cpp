@@ -52,36 +52,64 @@ void *workspace(size_t bytes)
     return c->ws;
 }
 
-/* Flags of the strip solves: at least `n` of them, and the first of
- * `launches` fresh epochs (flags hold the epoch of the launch that set them,
- * so they never need clearing between launches).  The solves run on the
- * main stream only, so one set of flags serves every call. */
-int *strip_flags(int n, int launches, int *epoch)
+/* ------------------------------------------------------------ lanes */
+Lane *main_lane() { return g_ctx ? &g_ctx->lane : nullptr; }
+
+hipEvent_t lane_event(Lane *L, size_t i)
 {
-    Context *c = g_ctx;
-    if (n > c->strip_flags_n || c->strip_epoch > (1 << 30)) {
-        CHX_HIP(hipStreamSynchronize(c->s_main));   /* no solve still reads the old flags */
-        if (c->strip_flags) CHX_HIP(hipFree(c->strip_flags));
-        const int cap = n > c->strip_flags_n ? n : c->strip_flags_n;
-        CHX_HIP(hipMalloc(&c->strip_flags, (size_t)cap * sizeof(int)));
-        CHX_HIP(hipMemsetAsync(c->strip_flags, 0, (size_t)cap * sizeof(int), c->s_main));
-        c->strip_flags_n = cap;
-        c->strip_epoch = 0;
+    while (L->events.size() <= i) {
+        hipEvent_t e;
+        CHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        L->events.push_back(e);
     }
-    *epoch = c->strip_epoch + 1;
-    c->strip_epoch += launches;
-    return c->strip_flags;
+    return L->events[i];
 }
 
-unsigned *panel_counters()
+/* Flags of the strip solves and the fused panel kernel: at least `n` of
+ * them, and the first of `launches` fresh epochs (flags hold the epoch of
+ * the launch that set them, so they never need clearing between launches).
+ * Every launch that uses them runs on the lane's critical stream, so one
+ * set serves every call on the lane. */
+int *lane_flags(Lane *L, int n, int launches, int *epoch)
 {
-    Context *c = g_ctx;
-    if (!c->panel_ctr) {
-        CHX_HIP(hipMalloc(&c->panel_ctr, 2 * sizeof(unsigned)));
-        CHX_HIP(hipMemsetAsync(c->panel_ctr, 0, 2 * sizeof(unsigned), c->s_main));
+    if (n > L->flags_n || L->epoch > (1 << 30)) {
+        CHX_HIP(hipStreamSynchronize(L->sm));   /* no launch still reads the old flags */
+        if (L->flags) CHX_HIP(hipFree(L->flags));
+        const int cap = n > L->flags_n ? n : L->flags_n;
+        CHX_HIP(hipMalloc(&L->flags, (size_t)cap * sizeof(int)));
+        k::zero_ints(L->flags, cap, L->sm);
+        L->flags_n = cap;
+        L->epoch = 0;
     }
-    return c->panel_ctr;
+    *epoch = L->epoch + 1;
+    L->epoch += launches;
+    return L->flags;
 }
+
+unsigned *lane_counters(Lane *L)
+{
+    if (!L->panel_ctr) {
+        CHX_HIP(hipMalloc(&L->panel_ctr, 2 * sizeof(unsigned)));
+        k::zero_ints((int *)L->panel_ctr, 2, L->sm);
+    }
+    return L->panel_ctr;
+}
+
+void lane_release(Lane *L)
+{
+    for (hipEvent_t e : L->events) CHX_HIP(hipEventDestroy(e));
+    L->events.clear();
+    if (L->flags) CHX_HIP(hipFree(L->flags));
+    if (L->panel_ctr) CHX_HIP(hipFree(L->panel_ctr));
+    L->flags = nullptr;
+    L->panel_ctr = nullptr;
+    L->flags_n = 0;
+    L->epoch = 0;
+}
+
+int *strip_flags(int n, int launches, int *epoch) { return lane_flags(&g_ctx->lane, n, launches, epoch); }
+
+unsigned *panel_counters() { return lane_counters(&g_ctx->lane); }
 
 void *workspace2(size_t bytes)
 {
@@ -352,6 +380,8 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     CHX_HIP(hipSetDevice(c->device));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking));
     CHX_HIP(hipStreamCreateWithFlags(&c->s_update, hipStreamNonBlocking));
+    c->lane.sm = c->s_main;
+    c->lane.su = c->s_update;
     CHX_HIP(hipMalloc(&c->d_info, sizeof(int)));
     CHX_HIP(hipMemset(c->d_info, 0, sizeof(int)));
     g_ctx = c;
@@ -369,13 +399,11 @@ int __chameleon_finalize(void)
     }
     CHX_HIP(hipDeviceSynchronize());
     latms_release();
-    for (hipEvent_t e : c->events) CHX_HIP(hipEventDestroy(e));
+    lane_release(&c->lane);
     stats_fold();
     for (hipEvent_t e : c->timing_pool) CHX_HIP(hipEventDestroy(e));
     if (c->ws) CHX_HIP(hipFree(c->ws));
     if (c->ws2) CHX_HIP(hipFree(c->ws2));
-    if (c->strip_flags) CHX_HIP(hipFree(c->strip_flags));
-    if (c->panel_ctr) CHX_HIP(hipFree(c->panel_ctr));
     if (c->pinned) CHX_HIP(hipHostFree(c->pinned));
     CHX_HIP(hipFree(c->d_info));
     CHX_HIP(hipStreamDestroy(c->s_main));

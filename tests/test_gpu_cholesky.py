@@ -400,3 +400,21 @@ def test_norms(ch):
     assert np.isclose(L.CHAMELEON_dlange(ch.FROBENIUS_NORM, n, 70, _p(G), n), np.linalg.norm(G), rtol=1e-13)
     for uplo in (L_, U_):
         assert np.isclose(L.CHAMELEON_dlansy(ch.ONE_NORM, uplo, n, _p(A), n), oracle.dlansy_one(A, uplo), rtol=1e-13)
+
+
+def test_norms_propagate_nan(ch):
+    """A NaN entry makes every norm NaN, as LAPACK's dlange/dlansy do (they
+    test each entry with disnan); a max that drops NaN would hide it."""
+    n, nb = 200, 64
+    say("norms nan")
+    ch.set_param(ch.TILE_SIZE, nb)
+    G = oracle.dplrnt(n, 70, 3)
+    G[137, 41] = np.nan
+    L = ch.lib
+    for norm in (ch.ONE_NORM, ch.INF_NORM, ch.MAX_NORM, ch.FROBENIUS_NORM):
+        assert np.isnan(L.CHAMELEON_dlange(norm, n, 70, _p(G), n)), norm
+    A = oracle.dplgsy(float(n), UL_, n, 51)
+    A[150, 20] = A[20, 150] = np.nan
+    for uplo in (L_, U_):
+        for norm in (ch.ONE_NORM, ch.MAX_NORM):
+            assert np.isnan(L.CHAMELEON_dlansy(norm, uplo, n, _p(A), n)), (uplo, norm)

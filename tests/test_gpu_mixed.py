@@ -242,7 +242,16 @@ def test_mixed_diagonal_scaling(ch, prec):
     ~1) and D spanning 1e-6 .. 1e6: the symmetric diagonal scaling hands the
     low-precision factorisation A0 itself, so refinement converges in every
     precision, FP8 included; without the scaling the answer is still the
-    FP64 one (converged or by fallback)."""
+    FP64 one (converged or by fallback).
+
+    The scaled matrix has a normwise condition number near 1e12 and
+    refinement stops at LAPACK's normwise test ||r|| <= ||x|| ||A|| eps
+    sqrt(N), which bounds the forward error only through that number.  With
+    the scaling the answer is held to 1e-10 of numpy's LU solution (6.4e-12
+    measured for FP32 products on MI355X); without it, refinement from an
+    FP8 factor of the unscaled matrix stops at that test further from it
+    (1.8e-9 measured), so 1e-7 there.  Both meet Chameleon's residual
+    check."""
     n, nb, nrhs = 1536, 256, 4
     A0 = oracle.dplgsy(float(n), UL_, n, 51)
     d = np.logspace(-6, 6, n)[np.random.default_rng(2).permutation(n)]
@@ -258,7 +267,7 @@ def test_mixed_diagonal_scaling(ch, prec):
             assert rc == 0
             if on:
                 assert 0 <= it <= 30, it
-            assert oracle.rel_frobenius(X, Xr) <= 1e-12
+            assert oracle.rel_frobenius(X, Xr) <= (1e-10 if on else 1e-7)
             assert oracle.check_solve(As, X, B)[0]
     finally:
         ch.lib.CHAMAMD_Set_Mixed_Scaling(1)
@@ -267,13 +276,18 @@ def test_mixed_diagonal_scaling(ch, prec):
 def test_mixed_fp8_nonfinite_falls_back(ch):
     """A NaN below the diagonal of the first panel reaches the FP8 demotion,
     whose range check stops the low-precision factorisation (ITER = -3);
-    the FP64 fallback then reports LAPACK's info for the same matrix."""
+    the FP64 fallback then reports the info of reference LAPACK for the same
+    matrix: dpotrf2/dpotf2 stop at the first pivot that is not positive or
+    is NaN (DISNAN), here column 301, which the oracle's textbook tile
+    kernels reproduce.  (OpenBLAS's own dpotrf, behind scipy, has no NaN
+    test and returns 0 on this matrix.)"""
     n, nb = 512, 128
     say("mixed fp8 nan")
     A = oracle.dplgsy(float(n), UL_, n, 51)
     A[300, 5] = A[5, 300] = np.nan
-    _, info_ref = __import__("scipy.linalg.lapack", fromlist=["dpotrf"]).dpotrf(A.copy(), lower=True)
     B = np.ones((n, 1), order="F")
+    info_ref = oracle.dposv(L_, A.copy(order="F"), n, B.copy(order="F"), 1, nb)
+    assert info_ref == 301
     rc, X, it, be = mixed(ch, L_, A.copy(order="F"), B, FP8, nb)
     assert it == -3
     assert rc == info_ref > 0
