@@ -36,6 +36,7 @@ TILE_SIZE, LOOKAHEAD = 2, 8
 SUCCESS = 0
 ERR_NOT_INITIALIZED = -101
 ERR_ILLEGAL_VALUE = -104
+ERR_OUT_OF_RESOURCES = -106
 COL_MAJOR = 102
 DIST_UNIFORM, DIST_SYMMETRIC, DIST_NORMAL = 201, 202, 203
 
@@ -107,6 +108,7 @@ def _declare(L):
         "CHAMELEON_dposv_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
         "CHAMELEON_dpotrf_Tile_Async": (i32, [i32, vp, vp, vp]),
         "CHAMELEON_dpotrs_Tile_Async": (i32, [i32, vp, vp, vp, vp]),
+        "CHAMAMD_dpotrs_Tile_Async_NoAlloc": (i32, [i32, vp, vp, vp, vp]),
         "CHAMELEON_dplgsy_Tile_Async": (i32, [dbl, i32, vp, u64, vp, vp]),
         "CHAMELEON_dplrnt_Tile_Async": (i32, [vp, u64, vp, vp]),
         "CHAMELEON_lapacke_dposv": (i32, [i32, ctypes.c_char, i32, i32, vp, i32, vp, i32]),

@@ -47,6 +47,9 @@ __host__ __device__ inline float nan_maxf(float a, float b)
 #define CHX_HIP(call) do { hipError_t _e = (call); if (_e != hipSuccess) ::chx::fatal_hip(_e, #call, __FILE__, __LINE__); } while (0)
 
 void latms_release();      /* frees the conditioned generator's kept factor */
+/* looks up the no-allocation solve's scratch array on `device` (current)
+ * and clears its flags on `s` (noalloc.hip) */
+void noalloc_init(int device, hipStream_t s);
 void error(const char *func, const char *msg);
 void warning(const char *func, const char *msg);
 

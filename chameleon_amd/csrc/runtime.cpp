@@ -382,6 +382,7 @@ int __chameleon_initpar(int nworkers, int ncudas, int nthreads_per_worker)
     CHX_HIP(hipStreamCreateWithFlags(&c->s_update, hipStreamNonBlocking));
     c->lane.sm = c->s_main;
     c->lane.su = c->s_update;
+    noalloc_init(c->device, c->s_main);
     CHX_HIP(hipMalloc(&c->d_info, sizeof(int)));
     CHX_HIP(hipMemset(c->d_info, 0, sizeof(int)));
     g_ctx = c;

@@ -201,6 +201,23 @@ int CHAMAMD_Server_Test(CHAMAMD_server_t *server, long long ticket, int *done, i
 int CHAMAMD_Server_Stats(CHAMAMD_server_t *server, CHAMAMD_server_stats_t *stats);
 int CHAMAMD_Server_Reset_Stats(CHAMAMD_server_t *server);
 
+/* dpotrs that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as CHAMELEON_dpotrs_Tile_Async, with the
+ * solve's scratch (64 x 64 diagonal-block inverses, transposed right-hand
+ * sides, publication flags) in a fixed device array of the library
+ * (32 MiB + 64 KiB per device, placed when CHAMELEON_Init loads the code
+ * object and held until the process ends) instead of allocated
+ * workspaces.  A and B must already be resident on the device (left there
+ * by an earlier _Tile_Async call), the padded tile size a multiple of 64,
+ * and N_padded * (64 + NRHS_padded) <= 4194304 (e.g. N = 32768 with
+ * NRHS <= 64); otherwise CHAMELEON_ERR_ILLEGAL_VALUE or
+ * CHAMELEON_ERR_OUT_OF_RESOURCES, nothing launched.  Calls that overlap in
+ * time on one device share the array and race. */
+int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B,
+                                      RUNTIME_sequence_t *sequence, RUNTIME_request_t *request);
+
 #ifdef __cplusplus
 }
 #endif
