@@ -24,6 +24,9 @@ Reported (one JSON line):
   solve_latency_ms  p50 / p99 of dpotrs with the factor resident in HBM
                (NRHS = 1 and 64, 100 calls each), the per-request cost of a
                repeated solve
+  lapack_layout the same dposv through the LAPACK-layout drop-in
+               CHAMELEON_dposv on column-major host arrays, transfers
+               included (end to end, the rate a LAPACK caller sees)
   mixed        for SPD matrices of condition number kappa (1e2, 1e4, 1e6;
                A = Q diag(kappa^(-i/(N-1))) Q^T, Q from the QR factorisation
                of a seeded Gaussian matrix, generated on the device), NRHS 1
@@ -37,11 +40,15 @@ Reported (one JSON line):
   batch        20 000 independent SPD systems, n = floor(exp(U(ln 16, ln 1024))),
                nrhs uniform in 1..8, A_i = dplgsy(bump = n_i, seed + i),
                B_i = dplrnt(seed + i), generated on the device: time of the
-               batched factor + solve and its flop rate
-  serve        Poisson arrivals (2000 requests/s for a few seconds), n log-uniform
-               in [64, 8192], nrhs in {1, 4, 16}, 30 % of requests reusing an
-               earlier matrix, 200 GB factor budget: sustained request rate and
-               flop rate, p50 / p99 latency from scheduled arrival to solution
+               batched factor + solve and its flop rate, FP64 and FP32
+               products + FP64 refinement, whole batch and per size bucket,
+               with the tuner's workgroup shapes
+  serve        Poisson arrivals (2000 requests/s, 10 s by default), n
+               log-uniform in [64, 8192], nrhs in {1, 4, 16}, 30 % of requests
+               reusing an earlier matrix, 200 GB factor budget, continuous
+               batching: sustained request rate and flop rate, p50 / p99
+               latency from scheduled arrival to solution, with FP64 factors
+               and with split-BF16 factors + FP64 refinement of every solve
 """
 import argparse
 import ctypes
@@ -264,46 +271,143 @@ def mixed_leg(ch, n, nb, kappas, nrhs_list=(1, 64), precs=("fp32", "bf16x3", "bf
     return res
 
 
-def batch_leg(ch, count=20000, reps=3, seed=2024):
-    """Uneven batch: every system factored and solved by one workgroup."""
+def lapack_leg(ch, n, nrhs, nb, reps=2):
+    """The LAPACK-layout drop-in on host arrays: CHAMELEON_dposv(Lower, N,
+    NRHS, A, LDA, B, LDB) with A and B column-major in (pageable) host
+    memory, as a LAPACK caller hands them over -- the transfers of A's
+    lower triangle and of B to the device, the factorisation and solve,
+    and the copies back of the factor and the solution, end to end.  A and
+    B are generated on the device and copied to the host untimed
+    (CHAMELEON_dplgsy / dplrnt, LAPACK layout).  Best of `reps`."""
     L = ch.lib
-    rng = np.random.default_rng(11)
-    n = np.floor(np.exp(rng.uniform(np.log(16), np.log(1024), count))).astype(np.int32)
-    nrhs = rng.integers(1, 9, count).astype(np.int32)
-    nn = n.astype(np.float64)
-    flops = float(np.sum(nn * (nn * nn / 3.0 + nn / 2.0 + 1.0 / 6.0) + 2.0 * nn * nn * nrhs))
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = np.zeros((n, n), order="F")
+    B = np.zeros((n, nrhs), order="F")
+    times = []
+    for _ in range(reps):
+        ch.check(L.CHAMELEON_dplgsy(float(n), ch.LOWER, n, ctypes.c_void_p(A.ctypes.data), n, 51))
+        ch.check(L.CHAMELEON_dplrnt(n, nrhs, ctypes.c_void_p(B.ctypes.data), n, 5673))
+        t0 = time.perf_counter()
+        info = L.CHAMELEON_dposv(ch.LOWER, n, nrhs, ctypes.c_void_p(A.ctypes.data), n,
+                                 ctypes.c_void_p(B.ctypes.data), n)
+        times.append(time.perf_counter() - t0)
+        if info != 0:
+            raise RuntimeError(f"CHAMELEON_dposv info={info}")
+        print(f"[bench] lapack-layout dposv n={n}: {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    best = min(times)
+    flops = flops_dpotrf(n) + flops_dpotrs(n, nrhs)
+    del A, B
+    return {"call": "CHAMELEON_dposv(Lower, N, NRHS, A, LDA=N, B, LDB=N), host column-major arrays",
+            "N": n, "NRHS": nrhs, "NB": nb, "ms": round(best * 1e3, 2), "tflops": round(flops / best / 1e12, 3),
+            "ms_all": [round(t * 1e3, 2) for t in times]}
+
+
+BUCKET_TOPS = (64, 128, 256, 512, 1024)
+
+
+def _batch_run(ch, n, nrhs, seed, reps, mixed):
+    """Best time of `reps` batched solves (FP64 dposv, or FP32-product
+    factorisation + FP64 refinement) after one untimed run; inputs
+    regenerated on the device before every run (untimed)."""
+    L = ch.lib
+    count = len(n)
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
     b = ctypes.c_void_p()
     ch.check(L.CHAMAMD_Batch_Create(ctypes.byref(b), count, ip(n), ip(nrhs)))
-    times, bad = [], 0
+    times, iters = [], None
     try:
         for it in range(reps + 1):
             ch.check(L.CHAMAMD_dplgsy_batch(b, seed))
             ch.check(L.CHAMAMD_dplrnt_batch(b, seed))
             info = np.zeros(count, dtype=np.int32)
+            itv = np.zeros(count, dtype=np.int32)
             t0 = time.perf_counter()
-            rc = L.CHAMAMD_dposv_batch(b, ip(info))
+            if mixed:
+                rc = L.CHAMAMD_dposv_mixed_batch(b, 0, ip(itv), ip(info))
+            else:
+                rc = L.CHAMAMD_dposv_batch(b, ip(info))
             t1 = time.perf_counter()
-            bad = int(np.count_nonzero(info))
-            if rc != 0 or bad:
-                raise RuntimeError(f"batch dposv rc={rc}, {bad} failing systems")
+            if rc != 0 or np.count_nonzero(info):
+                raise RuntimeError(f"batch solve rc={rc}, {np.count_nonzero(info)} failing systems")
             if it > 0:
                 times.append(t1 - t0)
-            print(f"[bench] batch {count} systems: {1e3 * (t1 - t0):.1f} ms", file=sys.stderr, flush=True)
+            iters = itv
     finally:
         L.CHAMAMD_Batch_Destroy(ctypes.byref(b))
-    best = min(times)
-    return {"systems": count, "n_range": [int(n.min()), int(n.max())], "flops": flops,
-            "ms": round(best * 1e3, 2), "tflops": round(flops / best / 1e12, 3),
-            "ms_all": [round(t * 1e3, 2) for t in times]}
+    return min(times), times, iters
 
 
-def serve_leg(ch, seconds, rate=2000.0, budget=200 << 30, seed=7, drain=20.0):
-    """Open-loop arrivals against the solve server; latency counted from
-    each request's scheduled arrival time (queueing included).  Arrivals
-    stop after `seconds`; serving stops `drain` seconds later and requests
-    still unserved then are reported (their latency is unknown, so the
-    percentiles are given only when fewer than 1 % are missing)."""
+def batch_leg(ch, count=20000, reps=3, seed=2024):
+    """Uneven batch (every system factored and solved by one workgroup,
+    the size buckets' workgroup shapes from CHAMAMD_Batch_Tune), in FP64
+    and with FP32-product factorisations + FP64 refinement, for the whole
+    batch and for the systems of each size bucket on their own."""
+    L = ch.lib
+    rng = np.random.default_rng(11)
+    n = np.floor(np.exp(rng.uniform(np.log(16), np.log(1024), count))).astype(np.int32)
+    nrhs = rng.integers(1, 9, count).astype(np.int32)
+
+    def flops_of(nn_, rr_):
+        nn_ = nn_.astype(np.float64)
+        return float(np.sum(nn_ * (nn_ * nn_ / 3.0 + nn_ / 2.0 + 1.0 / 6.0) + 2.0 * nn_ * nn_ * rr_))
+
+    flops = flops_of(n, nrhs)
+    # the tuner on this batch (untimed), its table reported
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    b = ctypes.c_void_p()
+    ch.check(L.CHAMAMD_Batch_Create(ctypes.byref(b), count, ip(n), ip(nrhs)))
+    table = (ctypes.c_int * 6)()
+    t0 = time.perf_counter()
+    ch.check(L.CHAMAMD_Batch_Tune(b, table))
+    tune_s = time.perf_counter() - t0
+    L.CHAMAMD_Batch_Destroy(ctypes.byref(b))
+    out = {"systems": count, "n_range": [int(n.min()), int(n.max())], "flops": flops,
+           "tuner": {"waves_per_system": dict(zip([f"<={t}" for t in BUCKET_TOPS] + [">1024"], list(table))),
+                     "tune_s": round(tune_s, 2)}}
+    for mode, mixed in (("fp64", False), ("fp32_ir", True)):
+        best, times, iters = _batch_run(ch, n, nrhs, seed, reps, mixed)
+        out[mode] = {"ms": round(best * 1e3, 2), "tflops": round(flops / best / 1e12, 3),
+                     "ms_all": [round(t * 1e3, 2) for t in times]}
+        if mixed:
+            out[mode]["ir_steps_max"] = int(iters.max())
+            out[mode]["ir_steps_mean"] = round(float(iters.mean()), 3)
+        print(f"[bench] batch {count} systems {mode}: {best * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    per = {}
+    lo = 0
+    for top in BUCKET_TOPS:
+        sel = (((n + 15) // 16) * 16 > lo) & (((n + 15) // 16) * 16 <= top)
+        lo = top
+        if not sel.any():
+            continue
+        nb_, rb_ = np.ascontiguousarray(n[sel]), np.ascontiguousarray(nrhs[sel])
+        fl = flops_of(nb_, rb_)
+        row = {"systems": int(sel.sum())}
+        for mode, mixed in (("fp64", False), ("fp32_ir", True)):
+            best, _, _ = _batch_run(ch, nb_, rb_, seed, 2, mixed)
+            row[mode] = {"ms": round(best * 1e3, 3), "tflops": round(fl / best / 1e12, 3)}
+        per[f"<={top}"] = row
+        print(f"[bench] batch bucket <={top}: {row}", file=sys.stderr, flush=True)
+    out["per_bucket"] = per
+    out["ms"] = out["fp64"]["ms"]
+    out["tflops"] = out["fp64"]["tflops"]
+    return out
+
+
+def serve_leg(ch, seconds, rate=2000.0, budget=200 << 30, seed=7, drain=30.0, prec=-1, nb=256):
+    """Open-loop Poisson arrivals against the solve server (continuous
+    batching: CHAMAMD_Server_Progress never blocks, work is admitted onto
+    free lanes while earlier work runs).  n log-uniform in [64, 8192],
+    nrhs in {1, 4, 16}, 30 % of requests reuse an earlier matrix (new
+    right-hand sides only), A = dplgsy(bump = n, seed) generated on the
+    device, 200 GB factor budget.  prec = -1: FP64 factors; else the
+    product precision of the large systems' factors (refined per request in
+    FP64).  Latency is counted from each request's scheduled arrival time
+    (queueing included).  Arrivals stop after `seconds`; serving stops
+    `drain` seconds later and requests still unserved then are reported
+    (their latency is unknown, so the percentiles are given only when fewer
+    than 1 % are missing)."""
+    L = ch.lib
+    ch.set_param(ch.TILE_SIZE, nb)
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 16))
     arrivals = arrivals[arrivals < seconds]
@@ -316,45 +420,56 @@ def serve_leg(ch, seconds, rate=2000.0, budget=200 << 30, seed=7, drain=20.0):
             mid = i
             seen.append((mid, n))
         reqs.append((float(arrivals[i]), mid, n, int(rng.choice([1, 4, 16]))))
+    # right-hand sides: copies of one random block (synthetic values)
+    base = np.asfortranarray(rng.uniform(-0.5, 0.5, (8192, 16)))
     srv = ch.Server(budget)
-    lat, done_at = [], {}
+    srv.set_precision(prec)
+    lat, live, bad = [], {}, 0
     try:
         t0 = time.perf_counter()
-        nxt, live = 0, {}
+        nxt = 0
         deadline = seconds + drain
         while (nxt < len(reqs) or live) and time.perf_counter() - t0 < deadline:
             now = time.perf_counter() - t0
             while nxt < len(reqs) and reqs[nxt][0] <= now:
                 ta, mid, n, r = reqs[nxt]
-                B = np.asfortranarray(rng.standard_normal((n, r)))
+                B = base[:n, :r].copy(order="F")
                 live[srv.submit(mid, n, B, seed=1000 + mid)] = ta
                 nxt += 1
-            if live:
-                srv.progress()
+            srv.progress()
+            done = srv.finished()
+            if done:
                 tdone = time.perf_counter() - t0
-                for t in list(live):
+                for t in done:
                     ok, st = srv.test(t)
-                    if ok:
-                        if st != 0:
-                            raise RuntimeError(f"server request status {st}")
-                        lat.append(tdone - live.pop(t))
-            elif nxt < len(reqs):
+                    if st != 0:
+                        bad += 1
+                    lat.append(tdone - live.pop(t))
+            elif not live and nxt < len(reqs):
                 time.sleep(max(0.0, reqs[nxt][0] - (time.perf_counter() - t0)))
         wall = time.perf_counter() - t0
         st = srv.stats()
     finally:
         srv.destroy()
+    if bad:
+        raise RuntimeError(f"server: {bad} requests failed")
     lat = np.array(lat) * 1e3
     served = len(lat)
     unserved = len(reqs) - served
     full = unserved <= 0.01 * len(reqs)
-    out = {"requests": len(reqs), "served": served, "unserved": unserved, "offered_rate": rate,
+    out = {"precision": "fp64" if prec < 0 else {0: "fp32", 1: "bf16", 2: "bf16x3", 3: "fp8"}[prec] + "+IR (n > 1024)",
+           "requests": len(reqs), "served": served, "unserved": unserved, "offered_rate": rate,
            "window_s": seconds, "wall_s": round(wall, 3), "served_rate": round(served / wall, 1),
            "tflops": round(st["flops"] / wall / 1e12, 3),
            "p50_ms": round(float(np.percentile(lat, 50)), 2) if served and full else None,
            "p99_ms": round(float(np.percentile(lat, 99)), 2) if served and full else None,
            "p50_ms_served": round(float(np.percentile(lat, 50)), 2) if served else None,
-           "factored": st["factored"], "reused": st["reused"], "resident": st["resident"]}
+           "p99_ms_served": round(float(np.percentile(lat, 99)), 2) if served else None,
+           "factored": st["factored"], "reused": st["reused"], "resident": st["resident"],
+           "max_lanes_busy": st["max_jobs"], "tile_size": nb}
+    if prec >= 0:
+        out.update({"mixed_solves": st["mixed_solves"], "ir_steps": st["ir_steps"], "fallbacks": st["fallbacks"],
+                    "max_berr": st["max_berr"]})
     print(f"[bench] serve: {out}", file=sys.stderr, flush=True)
     return out
 
@@ -487,13 +602,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--no-lapack", action="store_true", help="skip the LAPACK-layout (host arrays) dposv leg")
     ap.add_argument("--kappas", default="1e2,1e4,1e6", help="condition numbers of the mixed-precision leg")
     ap.add_argument("--mixed-nb", type=int, default=0, help="tile size of the mixed-precision leg (0: the FP64 one)")
     ap.add_argument("--mixed-precs", default="fp32,bf16x3,bf16,fp8", help="products of the mixed-precision leg")
     ap.add_argument("--mixed-nrhs", default="1,64", help="right-hand-side counts of the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--serve", type=float, default=3.0, metavar="SECONDS",
-                    help="arrival window of the serving leg (0: skip)")
+    ap.add_argument("--serve", type=float, default=10.0, metavar="SECONDS",
+                    help="arrival window of the serving legs (0: skip)")
+    ap.add_argument("--no-serve-mixed", dest="serve_mixed", action="store_false",
+                    help="skip the serving leg with mixed-precision factors")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run an N x N device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -531,12 +649,17 @@ def main():
                 "launches": up.get("launches"),
                 "avg_launch_ms": round(up["ms"] / up["launches"], 4) if up.get("launches") else None}
     slat = None if args.no_latency else solve_latency(ch, n, nb, 100)
+    lapack = None if args.no_lapack else lapack_leg(ch, n, nrhs, nb)
     mixed = None if args.no_mixed else mixed_leg(ch, n, args.mixed_nb or nb,
                                                  tuple(float(x) for x in args.kappas.split(",")),
                                                  tuple(int(x) for x in args.mixed_nrhs.split(",")),
                                                  tuple(args.mixed_precs.split(",")))
     batch = None if args.no_batch else batch_leg(ch)
-    serve = serve_leg(ch, args.serve) if args.serve > 0 else None
+    serve = None
+    if args.serve > 0:
+        serve = {"fp64": serve_leg(ch, args.serve)}
+        if args.serve_mixed:
+            serve["mixed"] = serve_leg(ch, args.serve, prec=2)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, nrhs, nb)
@@ -562,6 +685,7 @@ def main():
                         "tflops": round(v["flops"] / (v["ms"] * 1e-3) / 1e12, 3) if v["ms"] else None}
                     for k, v in kst.items()},
         "solve_latency_ms": slat,
+        "lapack_layout": lapack,
         "mixed": mixed,
         "batch": batch,
         "serve": serve,

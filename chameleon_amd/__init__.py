@@ -66,7 +66,9 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_longlong), ("rejected", ctypes.c_longlong), ("factored", ctypes.c_longlong),
                 ("reused", ctypes.c_longlong), ("resident", ctypes.c_longlong),
                 ("pool_used_bytes", ctypes.c_longlong), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
-                ("max_ms", ctypes.c_double), ("flops", ctypes.c_double)]
+                ("max_ms", ctypes.c_double), ("flops", ctypes.c_double), ("mixed_solves", ctypes.c_longlong),
+                ("ir_steps", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong), ("max_jobs", ctypes.c_longlong),
+                ("max_berr", ctypes.c_double)]
 
 
 def _declare(L):
@@ -147,10 +149,18 @@ def _declare(L):
         "CHAMAMD_dpotrs_batch_Async": (i32, [vp, vp, vp]),
         "CHAMAMD_dposv_batch_Async": (i32, [vp, vp, vp]),
         "CHAMAMD_Batch_Info": (i32, [vp, ip]),
+        "CHAMAMD_Batch_Tune": (i32, [vp, ip]),
+        "CHAMAMD_dposv_mixed_batch": (i32, [vp, i32, ip, ip]),
+        "CHAMAMD_Batch_Get_Tuning": (i32, [ip, ip, ip]),
+        "CHAMAMD_Batch_Set_Tuning": (i32, [ip]),
+        "CHAMAMD_Set_Batch_Graphs": (i32, [i32]),
         "CHAMAMD_Server_Create": (i32, [pp, ctypes.c_size_t]),
         "CHAMAMD_Server_Destroy": (i32, [pp]),
         "CHAMAMD_Server_Submit": (i32, [vp, ctypes.POINTER(ServerRequest), ctypes.POINTER(ctypes.c_longlong)]),
         "CHAMAMD_Server_Progress": (i32, [vp, ip]),
+        "CHAMAMD_Server_Drain": (i32, [vp]),
+        "CHAMAMD_Server_Finished": (i32, [vp, ctypes.POINTER(ctypes.c_longlong), i32, ip]),
+        "CHAMAMD_Server_Set_Precision": (i32, [vp, i32]),
         "CHAMAMD_Server_Test": (i32, [vp, ctypes.c_longlong, ip, ip]),
         "CHAMAMD_Server_Stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
         "CHAMAMD_Server_Reset_Stats": (i32, [vp]),
@@ -319,9 +329,30 @@ class Server:
         return t.value
 
     def progress(self):
+        """Non-blocking step; returns the number of requests finished."""
         n = ctypes.c_int()
         check(load().CHAMAMD_Server_Progress(self.ptr, ctypes.byref(n)), "CHAMAMD_Server_Progress")
         return n.value
+
+    def finished(self, cap=4096):
+        """Tickets of the requests finished since the last call."""
+        buf = (ctypes.c_longlong * cap)()
+        out = []
+        while True:
+            n = ctypes.c_int()
+            check(load().CHAMAMD_Server_Finished(self.ptr, buf, cap, ctypes.byref(n)), "CHAMAMD_Server_Finished")
+            out.extend(buf[:n.value])
+            if n.value < cap:
+                for t in out:
+                    self.keep.pop(t, None)
+                return out
+
+    def drain(self):
+        """Serve until every submitted request has finished."""
+        check(load().CHAMAMD_Server_Drain(self.ptr), "CHAMAMD_Server_Drain")
+
+    def set_precision(self, prec):
+        check(load().CHAMAMD_Server_Set_Precision(self.ptr, prec), "CHAMAMD_Server_Set_Precision")
 
     def test(self, ticket):
         d, st = ctypes.c_int(), ctypes.c_int()

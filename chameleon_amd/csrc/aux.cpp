@@ -79,6 +79,8 @@ double host_norm(CHAM_desc_t *A, cham_normtype_t norm, int uplo)
     if (!M || M->state == MIRROR_NONE) {
         mirror_acquire(A, MIRROR_NORMAL, false, s);
         M = mirror_of(A);
+    } else if (M->lower_only && uplo != ChamLower) {
+        mirror_complete(A, s);
     }
     const int trans = (M->state == MIRROR_TRANS);
     const size_t n = (size_t)A->m + A->n + 2;

@@ -21,6 +21,21 @@
  * kernels/batch.hip.  A system is computed by one workgroup in a fixed
  * order of operations, so its result does not depend on the rest of the
  * batch: solved alone or in any mix it is bitwise the same.
+ *
+ * Size buckets and tuning.  Systems fall in buckets by padded order
+ * (<= 64, 128, 256, 512, 1024, larger); each bucket is factored with the
+ * workgroup shape (4 or 8 wavefronts per system) of a process-wide table.
+ * The table starts from the shapes measured fastest on MI355X and
+ * CHAMAMD_Batch_Tune (run by itself on a batch's first factorisation while
+ * CHAMELEON_AUTOTUNING is on) times both shapes on a sample of every
+ * bucket the batch holds and keeps the faster.  Consecutive buckets of
+ * one shape go in one persistent launch; launches of the two shapes
+ * alternate between the two streams, so a bucket's launch fills the
+ * device while the previous one drains.
+ *
+ * Graphs.  The factorisation launches and the solve launch of a batch are
+ * captured once each (hipGraph; their queue counters are cleared by a
+ * kernel, not a memset node) and replayed by every later call.
  */
 #include "internal.h"
 #include "chameleon_amd.h"
@@ -32,8 +47,25 @@
 
 using namespace chx;
 
+#define NBUCKET 6
+static const int kBucketTop[NBUCKET] = {64, 128, 256, 512, 1024, 1 << 30};
+/* device counters of a batch: [0] unused, [1] solve queue, [2 .. 2+NBUCKET)
+ * factorisation queues, then the refinement's residual queue and its
+ * count of systems not yet converged */
+#define QUEUE_INTS (4 + 2 * NBUCKET)
+#define Q_RES (2 + 2 * NBUCKET)
+#define Q_UNCONV (3 + 2 * NBUCKET)
+
 struct CHAMAMD_batch_s {
     int count = 0;
+    int bstart[NBUCKET + 1] = {};              /* slots of bucket b: [bstart[b+1], bstart[b]) -- largest first */
+    hipGraphExec_t g_potrf = nullptr, g_potrs = nullptr;
+    /* mixed-precision solve: working factor, solution, residual, per system
+     * ||A||_inf, refinement state, steps and backward error */
+    double *dW = nullptr, *dX = nullptr, *dR = nullptr, *d_anrm = nullptr, *d_berr = nullptr;
+    int *d_state = nullptr, *d_iters = nullptr;
+    int g_waves[NBUCKET] = {};                 /* shapes the factorisation graph was captured with */
+    bool tuned = false;
     std::vector<int> slot;                     /* user index -> storage slot          */
     std::vector<int> user;                     /* slot -> user index                  */
     std::vector<int> n, np, nrhs;              /* per slot                            */
@@ -120,19 +152,205 @@ bool check_batch(const char *fn, const CHAMAMD_batch_t *b)
     return true;
 }
 
+int bucket_of(int np)
+{
+    int b = 0;
+    while (np > kBucketTop[b]) ++b;
+    return b;
+}
+
+/* Workgroup shape per bucket: wavefronts per system.  Defaults measured on
+ * MI355X (CHAMAMD_Batch_Tune on the 20 000-system log-uniform batch). */
+int g_bucket_waves[NBUCKET] = {4, 4, 4, 4, 4, 4};
+bool g_bucket_tuned[NBUCKET] = {};
+
+/* the factorisation launches of slots [s0, s1) with `waves`, of the
+ * matrices at base (the batch's A, or the mixed solve's working copy) */
+void potrf_range(CHAMAMD_batch_t *b, int s0, int s1, int waves, int q, hipStream_t s, double *base = nullptr,
+                 bool lp = false)
+{
+    if (s1 <= s0) return;
+    k::zero_ints(b->d_queue + q, 1, s);
+    k::potrf_jobs(base ? base : b->dA, b->d_aoff + s0, b->d_np + s0, nullptr, b->d_n + s0, b->dD, b->d_doff + s0,
+                  s1 - s0, b->d_queue + q, b->d_info + s0, 0, s, waves, lp);
+}
+
+/* Every bucket with its shape; runs of buckets with one shape in one
+ * launch; launches alternate between the main and the update stream
+ * (the update stream first waits for earlier main-stream work, and the
+ * main stream for the update stream's launches at the end). */
+void issue_potrf(CHAMAMD_batch_t *b, const int *waves, double *base = nullptr, bool lp = false)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main, su = c->s_update;
+    Lane *L = main_lane();
+    CHX_HIP(hipEventRecord(lane_event(L, 0), sm));
+    CHX_HIP(hipStreamWaitEvent(su, lane_event(L, 0), 0));
+    int launch = 0, bk = 0;
+    while (bk < NBUCKET) {
+        /* buckets are stored largest first: bucket NBUCKET-1 at slot 0 */
+        const int hb = NBUCKET - 1 - bk;
+        if (b->bstart[hb] == b->bstart[hb + 1]) { ++bk; continue; }
+        const int w = waves[hb];
+        int lo = hb;
+        while (lo - 1 >= 0 && (b->bstart[lo - 1] == b->bstart[lo] || waves[lo - 1] == w)) --lo;
+        const int s0 = b->bstart[hb + 1], s1 = b->bstart[lo];
+        potrf_range(b, s0, s1, w, 2 + launch, (launch & 1) ? su : sm, base, lp);
+        ++launch;
+        bk = NBUCKET - lo;
+    }
+    CHX_HIP(hipEventRecord(lane_event(L, 1), su));
+    CHX_HIP(hipStreamWaitEvent(sm, lane_event(L, 1), 0));
+}
+
+void issue_potrs(CHAMAMD_batch_t *b, hipStream_t s)
+{
+    k::zero_ints(b->d_queue + 1, 1, s);
+    k::batch_solve(b->dA, b->d_aoff, b->d_np, nullptr, b->dD, b->d_doff, b->dB, b->d_boff, b->d_nrhs, b->d_info,
+                   b->count, b->d_queue + 1, 0, s);
+}
+
+/* graph replay of the batch launches: CHAMAMD_BATCH_GRAPHS (default 1),
+ * CHAMAMD_Set_Batch_Graphs */
+int g_graphs = -1;
+
+bool graphs_on()
+{
+    if (g_graphs < 0) {
+        const char *e = getenv("CHAMAMD_BATCH_GRAPHS");
+        g_graphs = (e && *e) ? (atoi(e) != 0) : 1;
+    }
+    return g_graphs != 0;
+}
+
+void drop_graphs(CHAMAMD_batch_t *b)
+{
+    if (b->g_potrf) CHX_HIP(hipGraphExecDestroy(b->g_potrf));
+    if (b->g_potrs) CHX_HIP(hipGraphExecDestroy(b->g_potrs));
+    b->g_potrf = b->g_potrs = nullptr;
+}
+
+/* capture `issue` on the main stream into an executable graph */
+template <typename F>
+hipGraphExec_t capture(F issue)
+{
+    hipStream_t sm = ctx()->s_main;
+    k::potrf_jobs_prepare();
+    (void)k::batch_solve_resident();
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    CHX_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+    issue();
+    CHX_HIP(hipStreamEndCapture(sm, &g));
+    CHX_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    CHX_HIP(hipGraphDestroy(g));
+    return e;
+}
+
+int tune(CHAMAMD_batch_t *b, int *table);
+
 int batch_potrf(CHAMAMD_batch_t *b, hipStream_t s)
 {
-    k::zero_ints(b->d_queue, 1, s);
-    k::potrf_jobs(b->dA, b->d_aoff, b->d_np, nullptr, b->d_n, b->dD, b->d_doff, b->count, b->d_queue, b->d_info, 0,
-                  s);
+    (void)s;
+    if (ctx()->autotuning && !b->tuned) tune(b, nullptr);
+    if (!graphs_on()) {
+        issue_potrf(b, g_bucket_waves);
+        return CHAMELEON_SUCCESS;
+    }
+    if (b->g_potrf && memcmp(b->g_waves, g_bucket_waves, sizeof b->g_waves) != 0) {
+        CHX_HIP(hipGraphExecDestroy(b->g_potrf));
+        b->g_potrf = nullptr;
+    }
+    if (!b->g_potrf) {
+        memcpy(b->g_waves, g_bucket_waves, sizeof b->g_waves);
+        b->g_potrf = capture([&] { issue_potrf(b, b->g_waves); });
+    }
+    CHX_HIP(hipGraphLaunch(b->g_potrf, ctx()->s_main));
     return CHAMELEON_SUCCESS;
 }
 
 int batch_potrs(CHAMAMD_batch_t *b, hipStream_t s)
 {
-    k::zero_ints(b->d_queue + 1, 1, s);
-    k::batch_solve(b->dA, b->d_aoff, b->d_np, nullptr, b->dD, b->d_doff, b->dB, b->d_boff, b->d_nrhs, b->d_info,
-                   b->count, b->d_queue + 1, 0, s);
+    if (!graphs_on()) {
+        issue_potrs(b, s);
+        return CHAMELEON_SUCCESS;
+    }
+    if (!b->g_potrs) b->g_potrs = capture([&] { issue_potrs(b, ctx()->s_main); });
+    CHX_HIP(hipGraphLaunch(b->g_potrs, ctx()->s_main));
+    return CHAMELEON_SUCCESS;
+}
+
+/* Per bucket of this batch: time the factorisation of a sample of its
+ * systems (its largest, up to 1024 of them and 4 GB: two persistent grids'
+ * worth at the top bucket) with 4 and 8 wavefronts per system, twice each,
+ * and keep the faster shape (8 only when 3 % faster).  The sample is laid
+ * out like the batch in scratch memory and regenerated (dplgsy, bump = n)
+ * before every trial, so the batch's own matrices are not touched. */
+int tune(CHAMAMD_batch_t *b, int *table)
+{
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    CHX_HIP(hipStreamSynchronize(sm));
+    k::potrf_jobs_prepare();
+    hipEvent_t e0, e1;
+    CHX_HIP(hipEventCreate(&e0));
+    CHX_HIP(hipEventCreate(&e1));
+    for (int hb = 0; hb < NBUCKET; ++hb) {
+        const int s0 = b->bstart[hb + 1], s1 = b->bstart[hb];
+        if (s1 <= s0) continue;
+        int cnt = std::min(s1 - s0, 1024);
+        auto abytes = [&](int m) {
+            return (size_t)(b->aoff[s0 + m - 1] - b->aoff[s0] + (long long)b->np[s0 + m - 1] * b->np[s0 + m - 1]) *
+                   sizeof(double);
+        };
+        while (cnt > 1 && abytes(cnt) > ((size_t)4 << 30)) cnt = cnt * 3 / 4;
+        std::vector<long long> aoff(cnt), doff(cnt);
+        std::vector<int> np(cnt), nv(cnt);
+        std::vector<BatchGenSys> sys;
+        size_t d_elems = 0;
+        for (int i = 0; i < cnt; ++i) {
+            aoff[i] = b->aoff[s0 + i] - b->aoff[s0];
+            doff[i] = (long long)d_elems;
+            np[i] = b->np[s0 + i];
+            nv[i] = b->n[s0 + i];
+            d_elems += (size_t)((np[i] + 63) / 64) * 4096;
+            sys.push_back(gen_sys(aoff[i], np[i], np[i], np[i], nv[i], nv[i], 0, nv[i], 1ULL + i, (double)nv[i]));
+        }
+        double *tA = nullptr, *tD = nullptr;
+        int *tI = nullptr;
+        CHX_HIP(hipMalloc(&tA, abytes(cnt)));
+        CHX_HIP(hipMalloc(&tD, d_elems * sizeof(double)));
+        CHX_HIP(hipMalloc(&tI, (cnt + 1) * sizeof(int)));
+        long long *d_aoff = upload(aoff), *d_doff = upload(doff);
+        int *d_np = upload(np), *d_nv = upload(nv);
+        float t4 = 0.f;
+        int best_w = 4;
+        for (int w : {4, 8}) {
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                run_gen(tA, sys, 1, sm);
+                k::zero_ints(tI, cnt + 1, sm);
+                CHX_HIP(hipEventRecord(e0, sm));
+                k::potrf_jobs(tA, d_aoff, d_np, nullptr, d_nv, tD, d_doff, cnt, tI + cnt, tI, 0, sm, w);
+                CHX_HIP(hipEventRecord(e1, sm));
+                CHX_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                CHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                tmin = std::min(tmin, ms);
+            }
+            if (w == 4) t4 = tmin;
+            else if (tmin < 0.97f * t4) best_w = 8;
+        }
+        void *ptrs[] = {tA, tD, tI, d_aoff, d_doff, d_np, d_nv};
+        for (void *p : ptrs) CHX_HIP(hipFree(p));
+        g_bucket_waves[hb] = best_w;
+        g_bucket_tuned[hb] = true;
+    }
+    CHX_HIP(hipEventDestroy(e0));
+    CHX_HIP(hipEventDestroy(e1));
+    b->tuned = true;
+    if (table)
+        for (int i = 0; i < NBUCKET; ++i) table[i] = g_bucket_waves[i];
     return CHAMELEON_SUCCESS;
 }
 
@@ -195,6 +413,13 @@ int CHAMAMD_Batch_Create(CHAMAMD_batch_t **batch, int count, const int *n, const
         b->b_elems += (size_t)b->np[sl] * nrhs[u];
         b->d_elems += (size_t)((b->np[sl] + 63) / 64) * 4096;
     }
+    /* bucket boundaries over the slots (largest bucket first) */
+    for (int hb = 0; hb <= NBUCKET; ++hb) {
+        int cnt = 0;
+        for (int sl = 0; sl < count; ++sl)
+            if (hb < NBUCKET && bucket_of(b->np[sl]) >= hb) ++cnt;
+        b->bstart[hb] = cnt;
+    }
     if (count > 0) {
         CHX_HIP(hipMalloc(&b->dA, b->a_elems * sizeof(double)));
         CHX_HIP(hipMalloc(&b->dB, (b->b_elems ? b->b_elems : 1) * sizeof(double)));
@@ -208,7 +433,7 @@ int CHAMAMD_Batch_Create(CHAMAMD_batch_t **batch, int count, const int *n, const
         b->d_doff = upload(b->doff);
         CHX_HIP(hipMalloc(&b->d_info, count * sizeof(int)));
         CHX_HIP(hipMemset(b->d_info, 0, count * sizeof(int)));
-        CHX_HIP(hipMalloc(&b->d_queue, 2 * sizeof(int)));
+        CHX_HIP(hipMalloc(&b->d_queue, QUEUE_INTS * sizeof(int)));
         /* identity systems until the caller provides matrices */
         std::vector<BatchGenSys> sys;
         for (int sl = 0; sl < count; ++sl)
@@ -224,8 +449,9 @@ int CHAMAMD_Batch_Destroy(CHAMAMD_batch_t **batch)
     if (!batch || !*batch) return CHAMELEON_ERR_UNALLOCATED;
     CHAMAMD_batch_t *b = *batch;
     if (ctx()) CHX_HIP(hipStreamSynchronize(ctx()->s_main));
+    drop_graphs(b);
     void *ptrs[] = {b->dA, b->dB, b->dD, b->d_np, b->d_n, b->d_nrhs, b->d_info, b->d_queue, b->d_aoff, b->d_boff,
-                    b->d_doff};
+                    b->d_doff, b->dW, b->dX, b->dR, b->d_anrm, b->d_berr, b->d_state, b->d_iters};
     for (void *p : ptrs)
         if (p) CHX_HIP(hipFree(p));
     delete b;
@@ -362,6 +588,146 @@ int CHAMAMD_dpotrs_batch_Async(CHAMAMD_batch_t *b, RUNTIME_sequence_t *sequence,
 int CHAMAMD_dposv_batch_Async(CHAMAMD_batch_t *b, RUNTIME_sequence_t *sequence, RUNTIME_request_t *request)
 {
     return batch_async("CHAMAMD_dposv_batch_Async", b, sequence, request, true, true);
+}
+
+/* Mixed-precision solve of every system (LAPACK dsposv semantics per
+ * system): a working copy of A factored with FP32 products
+ * (potrf_jobs lp), X = W^-1 B, then FP64 refinement on the device --
+ * residual and dsposv's stopping test per system in one launch, one
+ * correction solve for the systems still refining -- at most 30 steps.
+ * Systems whose reduced-precision factorisation failed (ITER -3) or that
+ * did not converge (-31) are factored in FP64 in place and solved.  B
+ * receives the solutions; A is unchanged for the systems that converged
+ * and holds the FP64 factor for the others. */
+int CHAMAMD_dposv_mixed_batch(CHAMAMD_batch_t *b, int prec, int *iter, int *info)
+{
+    static const char *fn = "CHAMAMD_dposv_mixed_batch";
+    if (!check_batch(fn, b)) return CHAMELEON_ERR_ILLEGAL_VALUE;
+    if (prec != CHAMAMD_PREC_FP32) {
+        error(fn, "batches refine FP32-product factorisations only (prec = CHAMAMD_PREC_FP32)");
+        return CHAMELEON_ERR_ILLEGAL_VALUE;
+    }
+    if (b->count == 0) return CHAMELEON_SUCCESS;
+    Context *c = ctx();
+    hipStream_t sm = c->s_main;
+    const int count = b->count;
+    if (!b->dW) {
+        CHX_HIP(hipMalloc(&b->dW, b->a_elems * sizeof(double)));
+        CHX_HIP(hipMalloc(&b->dX, (b->b_elems ? b->b_elems : 1) * sizeof(double)));
+        CHX_HIP(hipMalloc(&b->dR, (b->b_elems ? b->b_elems : 1) * sizeof(double)));
+        CHX_HIP(hipMalloc(&b->d_anrm, count * sizeof(double)));
+        CHX_HIP(hipMalloc(&b->d_berr, count * sizeof(double)));
+        CHX_HIP(hipMalloc(&b->d_state, count * sizeof(int)));
+        CHX_HIP(hipMalloc(&b->d_iters, count * sizeof(int)));
+    }
+    CHX_HIP(hipMemcpyAsync(b->dW, b->dA, b->a_elems * sizeof(double), hipMemcpyDeviceToDevice, sm));
+    if (b->b_elems) CHX_HIP(hipMemcpyAsync(b->dX, b->dB, b->b_elems * sizeof(double), hipMemcpyDeviceToDevice, sm));
+    issue_potrf(b, g_bucket_waves, b->dW, true);
+    k::zero_ints(b->d_queue + 1, 1, sm);
+    k::batch_solve(b->dW, b->d_aoff, b->d_np, nullptr, b->dD, b->d_doff, b->dX, b->d_boff, b->d_nrhs, b->d_info,
+                   count, b->d_queue + 1, 0, sm);
+    k::batch_refine_init(b->d_info, b->d_state, b->d_iters, b->d_anrm, count, sm);
+    for (int it = 0;; ++it) {
+        k::zero_ints(b->d_queue + Q_RES, 2, sm);
+        k::batch_residual(b->dA, b->d_aoff, b->d_np, b->d_n, b->dB, b->dX, b->dR, b->d_boff, b->d_nrhs, b->d_anrm,
+                          b->d_state, b->d_iters, b->d_queue + Q_UNCONV, b->d_berr, count, b->d_queue + Q_RES, sm);
+        int unconv = 0;
+        CHX_HIP(hipMemcpyAsync(&unconv, b->d_queue + Q_UNCONV, sizeof(int), hipMemcpyDeviceToHost, sm));
+        CHX_HIP(hipStreamSynchronize(sm));
+        if (unconv == 0 || it == 30) break;
+        /* correction of the systems still refining (state 0) */
+        k::zero_ints(b->d_queue + 1, 1, sm);
+        k::batch_solve(b->dW, b->d_aoff, b->d_np, nullptr, b->dD, b->d_doff, b->dR, b->d_boff, b->d_nrhs,
+                       b->d_state, count, b->d_queue + 1, 0, sm);
+        k::batch_update(b->dX, b->dR, b->d_boff, b->d_np, b->d_nrhs, b->d_state, count, 1, sm);
+    }
+    std::vector<int> state(count), iters(count);
+    CHX_HIP(hipMemcpy(state.data(), b->d_state, count * sizeof(int), hipMemcpyDeviceToHost));
+    CHX_HIP(hipMemcpy(iters.data(), b->d_iters, count * sizeof(int), hipMemcpyDeviceToHost));
+    /* FP64 fallback of the systems that did not converge */
+    std::vector<long long> aoff, doff;
+    std::vector<int> np, nv, slot;
+    for (int sl = 0; sl < count; ++sl)
+        if (state[sl] != 1) {
+            aoff.push_back(b->aoff[sl]);
+            doff.push_back(b->doff[sl]);
+            np.push_back(b->np[sl]);
+            nv.push_back(b->n[sl]);
+            slot.push_back(sl);
+        }
+    std::vector<int> finfo(count, 0);
+    if (!slot.empty()) {
+        const int nf = (int)slot.size();
+        long long *d_aoff = upload(aoff), *d_doff = upload(doff);
+        int *d_np = upload(np), *d_nv = upload(nv);
+        std::vector<long long> boff;
+        std::vector<int> nr;
+        for (int sl : slot) {
+            boff.push_back(b->boff[sl]);
+            nr.push_back(b->nrhs[sl]);
+        }
+        long long *d_boff = upload(boff);
+        int *d_nr = upload(nr), *d_fi = nullptr;
+        CHX_HIP(hipMalloc(&d_fi, (nf + 2) * sizeof(int)));
+        k::zero_ints(d_fi, nf + 2, sm);
+        k::potrf_jobs(b->dA, d_aoff, d_np, nullptr, d_nv, b->dD, d_doff, nf, d_fi + nf, d_fi, 0, sm);
+        k::batch_update(b->dX, b->dB, b->d_boff, b->d_np, b->d_nrhs, b->d_state, count, 0, sm);
+        k::batch_solve(b->dA, d_aoff, d_np, nullptr, b->dD, d_doff, b->dX, d_boff, d_nr, d_fi, nf, d_fi + nf + 1, 0,
+                       sm);
+        std::vector<int> fi(nf);
+        CHX_HIP(hipMemcpyAsync(fi.data(), d_fi, nf * sizeof(int), hipMemcpyDeviceToHost, sm));
+        CHX_HIP(hipStreamSynchronize(sm));
+        for (int i = 0; i < nf; ++i) finfo[slot[i]] = fi[i];
+        void *ptrs[] = {d_aoff, d_doff, d_np, d_nv, d_boff, d_nr, d_fi};
+        for (void *p : ptrs) CHX_HIP(hipFree(p));
+    }
+    if (b->b_elems) CHX_HIP(hipMemcpyAsync(b->dB, b->dX, b->b_elems * sizeof(double), hipMemcpyDeviceToDevice, sm));
+    /* the per-system info the batch reports next: the FP64 fallback's */
+    CHX_HIP(hipMemcpyAsync(b->d_info, finfo.data(), count * sizeof(int), hipMemcpyHostToDevice, sm));
+    CHX_HIP(hipStreamSynchronize(sm));
+    int status = 0;
+    for (int u = 0; u < count; ++u) {
+        const int sl = b->slot[u];
+        if (iter) iter[u] = state[sl] == 1 ? iters[sl] : (state[sl] == 2 ? -3 : -31);
+        if (info) info[u] = finfo[sl];
+        if (finfo[sl] != 0 && status == 0) status = finfo[sl];
+    }
+    return status;
+}
+
+int CHAMAMD_Set_Batch_Graphs(int on)
+{
+    if (!ctx()) return CHAMELEON_ERR_NOT_INITIALIZED;
+    g_graphs = on ? 1 : 0;
+    return CHAMELEON_SUCCESS;
+}
+
+int CHAMAMD_Batch_Tune(CHAMAMD_batch_t *b, int *waves)
+{
+    if (!check_batch("CHAMAMD_Batch_Tune", b)) return CHAMELEON_ERR_ILLEGAL_VALUE;
+    if (b->count == 0) return CHAMELEON_SUCCESS;
+    return tune(b, waves);
+}
+
+int CHAMAMD_Batch_Get_Tuning(int *waves, int *bucket_top, int *tuned)
+{
+    if (!ctx()) return CHAMELEON_ERR_NOT_INITIALIZED;
+    for (int i = 0; i < NBUCKET; ++i) {
+        if (waves) waves[i] = g_bucket_waves[i];
+        if (bucket_top) bucket_top[i] = kBucketTop[i];
+        if (tuned) tuned[i] = g_bucket_tuned[i];
+    }
+    return CHAMELEON_SUCCESS;
+}
+
+int CHAMAMD_Batch_Set_Tuning(const int *waves)
+{
+    if (!ctx()) return CHAMELEON_ERR_NOT_INITIALIZED;
+    if (!waves) return CHAMELEON_ERR_ILLEGAL_VALUE;
+    for (int i = 0; i < NBUCKET; ++i)
+        if (waves[i] != 4 && waves[i] != 8) return CHAMELEON_ERR_ILLEGAL_VALUE;
+    for (int i = 0; i < NBUCKET; ++i) g_bucket_waves[i] = waves[i];
+    return CHAMELEON_SUCCESS;
 }
 
 int CHAMAMD_Batch_Info(CHAMAMD_batch_t *b, int *info)

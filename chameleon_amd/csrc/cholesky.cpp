@@ -262,7 +262,7 @@ static void solve_pair(const SolvePlan &p, double *A, int NT, int ldt, double *B
     /* 64 x 64 inverse of block b of diagonal tile t at dinv + (t*nob + b)*4096, ld 64 */
     for (int dir = 0; dir < 2; ++dir)
         k::solve_strips(A, p.dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
-                        d_info, sm);
+                        panel_counters() + 2, d_info, sm);
 }
 
 /* In-place solve of transposed right-hand sides BT (nrp x NT*ldt) with the
@@ -281,7 +281,7 @@ void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int
     const size_t dper = (size_t)(ldt / 64) * 64 * 64;
     for (int dir = 0; dir < 2; ++dir)
         k::solve_strips(A, dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
-                        d_info, L->sm);
+                        lane_counters(L) + 2, d_info, L->sm);
 }
 
 /* B <- A^-1 B with the Lower factor in A: forward then backward solve on
@@ -347,7 +347,7 @@ int potrf_async(cham_uplo_t uplo, CHAM_desc_t *A, RUNTIME_sequence_t *sequence)
 {
     Context *c = ctx();
     if (A->m == 0) return CHAMELEON_SUCCESS;
-    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main);
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main, uplo == ChamLower);
     Mirror *M = mirror_of(A);
     SeqState *st = seq_state(sequence);
     PotrfPlan p;
@@ -368,7 +368,7 @@ int potrs_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequen
 {
     Context *c = ctx();
     if (A->m == 0 || B->n == 0) return CHAMELEON_SUCCESS;
-    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main);
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main, uplo == ChamLower);
     mirror_acquire(B, MIRROR_NORMAL, false, c->s_main);
     Mirror *MA = mirror_of(A), *MB = mirror_of(B);
     SeqState *st = seq_state(sequence);
@@ -478,9 +478,9 @@ void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate)
     const double n = (double)(NT - 1) * p.nb + p.nvalid_last, rows_left = n - (double)f.next * p.nb;
     timed(CHAMAMD_KERNEL_SOLVE, (rows_left * rows_left + n * n) * nrhs, sm, [&] {
         k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 0, flags, epoch,
-                        p.d_info, sm, s0);
+                        lane_counters(L) + 2, p.d_info, sm, s0);
         k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 1, flags,
-                        epoch + chunks, p.d_info, sm);
+                        epoch + chunks, lane_counters(L) + 2, p.d_info, sm);
     });
 }
 
@@ -490,7 +490,7 @@ int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequenc
     hipStream_t sm = c->s_main;
     if (A->m == 0) return CHAMELEON_SUCCESS;
     if (B->n == 0) return potrf_async(uplo, A, sequence);
-    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, sm);
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, sm, uplo == ChamLower);
     mirror_acquire(B, MIRROR_NORMAL, false, sm);
     Mirror *MA = mirror_of(A), *MB = mirror_of(B);
     if (!(potrf_solve_eligible(MA->ldt) && MB->ldt == MA->ldt)) {

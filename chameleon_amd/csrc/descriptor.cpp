@@ -390,3 +390,5 @@ int desc_check(const CHAM_desc_t *d)
 }
 
 } /* namespace chx */
+
+bool chx::desc_is_cm(const CHAM_desc_t *A) { return A->get_blkaddr == getaddr_cm; }

@@ -65,6 +65,8 @@ struct Mirror {
     int     state = MIRROR_NONE;  /* content valid on device, orientation   */
     bool    host_valid = true;    /* host copy up to date                   */
     bool    diag_identity = false;/* padding of diagonal tiles is identity  */
+    bool    lower_only = false;   /* only tiles m >= n hold the matrix (the
+                                     upper ones were never uploaded)       */
 };
 
 /* Where a factorisation or a strip solve runs: the critical-path stream,
@@ -86,6 +88,8 @@ Lane *main_lane();
 hipEvent_t lane_event(Lane *L, size_t i);
 /* at least n flags and the first of `launches` fresh epochs */
 int *lane_flags(Lane *L, int n, int launches, int *epoch);
+/* four counters: [0..1] the fused panel kernel's tickets, [2] the strip
+ * solves' task ticket */
 unsigned *lane_counters(Lane *L);
 void lane_release(Lane *L);         /* frees flags, counters and events (not the streams) */
 
@@ -153,8 +157,13 @@ unsigned *panel_counters();
 
 Mirror *mirror_of(const CHAM_desc_t *A);
 /* Make the device mirror of A hold its current values in orientation
- * `state` (MIRROR_NORMAL or MIRROR_TRANS); uploads from host if needed. */
-int mirror_acquire(CHAM_desc_t *A, int state, bool diag_identity, hipStream_t s);
+ * `state` (MIRROR_NORMAL or MIRROR_TRANS); uploads from host if needed.
+ * lower_need: the caller reads and writes only the tiles m >= n (a Lower
+ * Cholesky), so a fresh upload of a column-major matrix may move those
+ * only; any later full acquire uploads the rest. */
+int mirror_acquire(CHAM_desc_t *A, int state, bool diag_identity, hipStream_t s, bool lower_need = false);
+/* a mirror holding the lower tiles only gets the others from the host */
+void mirror_complete(CHAM_desc_t *A, hipStream_t s);
 /* Allocate the mirror (without uploading) for a matrix that the next
  * operation overwrites entirely. */
 int mirror_prepare_overwrite(CHAM_desc_t *A, int state, bool diag_identity);
@@ -165,6 +174,16 @@ int mirror_flush(CHAM_desc_t *A, hipStream_t s);
 void mirror_release(CHAM_desc_t *A);
 
 int desc_check(const CHAM_desc_t *A);
+bool desc_is_cm(const CHAM_desc_t *A);     /* user column-major storage (CM) */
+
+/* transfer.cpp: column-panel transfers of column-major descriptors */
+#define XFER_ALL 0
+#define XFER_LOWER 1
+#define XFER_UPPER 2
+bool xfer_eligible(const CHAM_desc_t *A, const Mirror *M);
+void xfer_upload(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s);
+void xfer_download(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s);
+void xfer_release();
 int request_fail(RUNTIME_sequence_t *seq, RUNTIME_request_t *req, int status);
 
 /* sequences: device info slot for potrf of this sequence */
@@ -270,8 +289,16 @@ int update_pairs(int n0, int n1, int NT, int *ncol);
 void trailing_update(double *A, int lmt, int ldt, int k, int n0, int n1, int NT, const int *info, hipStream_t s,
                      int kp = 1, const unsigned char *pmap = nullptr);
 void dinv_all(const double *A, int nt, size_t diag_stride, int ldt, double *dinv, hipStream_t s);
+/* waves: 4 or 8 wavefronts per workgroup (one workgroup per job); lp: the
+ * left-looking updates multiply FP32 operands with FP32 accumulation (the
+ * 64 x 64 diagonal blocks, their inverses and the solves below stay FP64) */
 void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, const int *nvalid, double *dinv,
-                const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s);
+                const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s, int waves = 4,
+                bool lp = false);
+/* occupancy queries of the persistent batch kernels, made before any
+ * stream capture (they are host calls, not stream work) */
+void potrf_jobs_prepare();
+int batch_solve_resident();
 /* batch.hip */
 void batch_gen(double *A, const BatchGenSys *sys, const long long *unit0, long long units, int count, int sym,
                hipStream_t s);
@@ -279,8 +306,19 @@ void batch_solve(const double *A, const long long *aoff, const int *np, const in
                  const long long *doff, double *B, const long long *boff, const int *nrhs, const int *info, int count,
                  int *queue, int grid, hipStream_t s);
 void zero_ints(int *p, int n, hipStream_t s);
+/* refinement pieces of the batched mixed-precision solve (batch.hip) */
+void batch_residual(const double *A, const long long *aoff, const int *np, const int *n, const double *B,
+                    const double *X, double *R, const long long *boff, const int *nrhs, double *anrm, int *state,
+                    int *iters, int *unconv, double *berr, int count, int *queue, hipStream_t s);
+void batch_update(double *X, const double *R, const long long *boff, const int *np, const int *nrhs,
+                  const int *state, int count, int add, hipStream_t s);
+void batch_refine_init(const int *info, int *state, int *iters, double *anrm, int count, hipStream_t s);
 void tile_pad_diag(double *d, int mt, int nt, int ldt, int wdt, int mb, int nb, int M, int N, int mode,
                    hipStream_t s);
+/* dense rows x cols column panel (rows r0.., columns c0.. of tile column n)
+ * to (to_tiles = 1) or from (0) the tiles of a mirror */
+void panel_tiles(double *P, double *D, int mt, int ldt, size_t tile_elems, int mb, int n, int r0, int rows, int c0,
+                 int cols, int to_tiles, hipStream_t s);
 /* mixed.hip */
 void panel_demote(const double *A, int NT, int ldt, int k, int m0, int m1, void *P, int mode, hipStream_t s);
 /* FP8: tiles m0..m1-1 of panel k as E4M3 bytes with a power-of-two scale
@@ -318,8 +356,11 @@ void sumsq_tiles(const double *A, int ntiles, int ldt, double *part, hipStream_t
 void tile_gemm_nt(const double *A, int amt, const double *B, int bmt, double *C, int cmt, int ldt, int k0, int k1,
                   int r0, int nr, int c0, int nc, int lower, double alpha, double beta, hipStream_t s);
 /* solve.hip */
+/* ticket: one device counter, cleared by the call before each launch and
+ * used by that launch only (launches on one stream may share it) */
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
-                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s, int s0 = 0);
+                 int ldt, int nrp, int trans, int *flags, int epoch, unsigned *ticket, const int *info, hipStream_t s,
+                 int s0 = 0);
 void solve_coop(const double *A, const double *TN, const double *TT, double *BT, double *W, int NT, int ldt, int nrhs,
                 int trans, const int *info, hipStream_t s);
 /* with info != null, nothing is copied once *info is nonzero */

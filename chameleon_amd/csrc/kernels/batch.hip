@@ -9,6 +9,8 @@
  * the 64 x 64 inverses of its diagonal blocks, all at per-system offsets.
  */
 #include "../internal.h"
+
+#include <algorithm>
 #include "mfma.h"
 #include "rng.h"
 
@@ -173,27 +175,248 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(BatchSolveArgs a)
     }
 }
 
+int batch_solve_resident()
+{
+    static int resident = 0;
+    if (!resident) {
+        int dev = 0, ncu = 0, per = 0;
+        CHX_HIP(hipGetDevice(&dev));
+        CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_solve_kernel, 256, 0));
+        resident = ncu * (per > 0 ? per : 1);
+    }
+    return resident;
+}
+
 void batch_solve(const double *A, const long long *aoff, const int *np, const int *ld, const double *dinv,
                  const long long *doff, double *B, const long long *boff, const int *nrhs, const int *info, int count,
                  int *queue, int grid, hipStream_t s)
 {
     if (count <= 0) return;
-    if (grid <= 0) {
-        static int resident = 0;
-        if (!resident) {
-            int dev = 0, ncu = 0, per = 0;
-            CHX_HIP(hipGetDevice(&dev));
-            CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_solve_kernel, 256, 0));
-            resident = ncu * (per > 0 ? per : 1);
-        }
-        grid = resident;
-    }
+    if (grid <= 0) grid = batch_solve_resident();
     if (grid > count) grid = count;
     BatchSolveArgs a;
     a.A = A; a.aoff = aoff; a.np = np; a.ld = ld; a.dinv = dinv; a.doff = doff;
     a.B = B; a.boff = boff; a.nrhs = nrhs; a.info = info; a.count = count; a.queue = queue;
     hipLaunchKernelGGL(batch_solve_kernel, dim3(grid), dim3(256), 0, s, a);
+}
+
+/* ------------------------------------------------------------------ */
+/* Refinement pieces of the batched mixed-precision solve.
+ *
+ * batch_residual_kernel: for every system j still refining (state[j] == 0)
+ *   R = B - A X in FP64, A symmetric with its lower triangle stored
+ *   (np x np, leading dimension np), then LAPACK dsposv's test per
+ *   right-hand side c:  ||R_c||_inf <= ||X_c||_inf ||A||_inf eps sqrt(n);
+ *   all pass -> state[j] = 1, else iters[j] += 1 and *unconv += 1.  The
+ *   backward error max_c ||R_c|| / (||A|| ||X_c|| n eps) goes to berr[j].
+ *   ||A||_inf (largest row sum of |A|) is formed in the same pass when
+ *   anrm[j] < 0.
+ * One workgroup of 256 threads per system (queue); thread t owns rows
+ * t, t + 256, ...: row i takes A(i, 0..i) (consecutive threads read
+ * consecutive rows: coalesced) and A(i+1.., i) down its own column, n
+ * elements for every row.  Right-hand sides go in chunks of 8 columns;
+ * the chunk's X rows are staged in LDS 1024 rows at a time (row-major, 8
+ * wide).  Every sum runs in a fixed order: deterministic. */
+struct BatchResArgs {
+    const double *A;
+    const long long *aoff;
+    const int *np, *n;
+    const double *B, *X;
+    double *R;
+    const long long *boff;
+    const int *nrhs;
+    double *anrm;
+    int *state, *iters, *unconv;
+    double *berr;
+    int count;
+    int *queue;
+};
+
+__global__ __launch_bounds__(256) void batch_residual_kernel(BatchResArgs a)
+{
+    constexpr int JB = 1024, CW = 8, RG = 1024;
+    __shared__ double Xs[JB * CW];
+    __shared__ double red[256];
+    __shared__ double colr[CW], colx[CW];
+    __shared__ int s_job;
+    const int tid = threadIdx.x;
+    const double eps = 0x1p-53;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_job = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int j = s_job;
+        if (j >= a.count) break;
+        if (a.state[j] != 0) continue;
+        const double *A = a.A + a.aoff[j];
+        const int np = a.np[j], n = a.n[j], nr = a.nrhs[j];
+        const double *B = a.B + a.boff[j], *X = a.X + a.boff[j];
+        double *R = a.R + a.boff[j];
+        const bool want_norm = a.anrm[j] < 0.0;
+        double an = want_norm ? 0.0 : a.anrm[j];
+        double worst = 0.0;
+        bool ok = true;
+        for (int c0 = 0; c0 < nr; c0 += CW) {
+            const int cw = min(CW, nr - c0);
+            const bool norm_pass = want_norm && c0 == 0;
+            double rmax[CW], xmax[CW], rowmax = 0.0;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) rmax[c] = xmax[c] = 0.0;
+            for (int rg = 0; rg < n; rg += RG) {
+                double acc[RG / 256][CW], rowabs[RG / 256];
+#pragma unroll
+                for (int q = 0; q < RG / 256; ++q) {
+                    rowabs[q] = 0.0;
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) acc[q][c] = 0.0;
+                }
+                for (int jb = 0; jb < n; jb += JB) {
+                    const int je = min(n, jb + JB), len = je - jb;
+                    __syncthreads();
+                    for (int e = tid; e < len * CW; e += 256) {
+                        const int c = e / len, jj = e - c * len;
+                        Xs[jj * CW + c] = c < cw ? X[(jb + jj) + (size_t)(c0 + c) * np] : 0.0;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < RG / 256; ++q) {
+                        const int i = rg + tid + 256 * q;
+                        if (i >= n) break;
+                        /* row part: A(i, jb .. min(i, je-1)) */
+                        const int rend = min(i + 1, je);
+                        for (int jj = jb; jj < rend; ++jj) {
+                            const double v = A[i + (size_t)jj * np];
+                            if (norm_pass) rowabs[q] += fabs(v);
+                            const double *xs = &Xs[(jj - jb) * CW];
+#pragma unroll
+                            for (int c = 0; c < CW; ++c) acc[q][c] += v * xs[c];
+                        }
+                        /* column part: A(max(i+1, jb) .. je-1, i) */
+                        const double *col = A + (size_t)i * np;
+                        for (int jj = max(i + 1, jb); jj < je; ++jj) {
+                            const double v = col[jj];
+                            if (norm_pass) rowabs[q] += fabs(v);
+                            const double *xs = &Xs[(jj - jb) * CW];
+#pragma unroll
+                            for (int c = 0; c < CW; ++c) acc[q][c] += v * xs[c];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < RG / 256; ++q) {
+                    const int i = rg + tid + 256 * q;
+                    if (i >= n) break;
+                    rowmax = nan_max(rowmax, rowabs[q]);
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        if (c >= cw) break;
+                        const size_t o = i + (size_t)(c0 + c) * np;
+                        const double r = B[o] - acc[q][c];
+                        R[o] = r;
+                        rmax[c] = nan_max(rmax[c], fabs(r));
+                        xmax[c] = nan_max(xmax[c], fabs(X[o]));
+                    }
+                }
+            }
+            /* workgroup maxima: ||A||_inf (first chunk), then per column */
+            if (norm_pass) {
+                __syncthreads();
+                red[tid] = rowmax;
+                __syncthreads();
+                for (int w = 128; w > 0; w >>= 1) {
+                    if (tid < w) red[tid] = nan_max(red[tid], red[tid + w]);
+                    __syncthreads();
+                }
+                an = red[0];
+                if (tid == 0) a.anrm[j] = an;
+            }
+            for (int c = 0; c < cw; ++c) {
+                __syncthreads();
+                red[tid] = rmax[c];
+                __syncthreads();
+                for (int w = 128; w > 0; w >>= 1) {
+                    if (tid < w) red[tid] = nan_max(red[tid], red[tid + w]);
+                    __syncthreads();
+                }
+                if (tid == 0) colr[c] = red[0];
+                __syncthreads();
+                red[tid] = xmax[c];
+                __syncthreads();
+                for (int w = 128; w > 0; w >>= 1) {
+                    if (tid < w) red[tid] = nan_max(red[tid], red[tid + w]);
+                    __syncthreads();
+                }
+                if (tid == 0) colx[c] = red[0];
+            }
+            __syncthreads();
+            for (int c = 0; c < cw; ++c) {
+                if (!(colr[c] <= colx[c] * an * eps * sqrt((double)n))) ok = false;
+                const double d = an * colx[c] * n * eps;
+                worst = nan_max(worst, d > 0.0 ? colr[c] / d : (colr[c] > 0.0 ? INFINITY : 0.0));
+            }
+        }
+        if (tid == 0) {
+            a.berr[j] = worst;
+            if (ok) {
+                a.state[j] = 1;
+            } else {
+                a.iters[j] += 1;
+                atomicAdd(a.unconv, 1);
+            }
+        }
+    }
+}
+
+void batch_residual(const double *A, const long long *aoff, const int *np, const int *n, const double *B,
+                    const double *X, double *R, const long long *boff, const int *nrhs, double *anrm, int *state,
+                    int *iters, int *unconv, double *berr, int count, int *queue, hipStream_t s)
+{
+    if (count <= 0) return;
+    BatchResArgs a;
+    a.A = A; a.aoff = aoff; a.np = np; a.n = n; a.B = B; a.X = X; a.R = R; a.boff = boff; a.nrhs = nrhs;
+    a.anrm = anrm; a.state = state; a.iters = iters; a.unconv = unconv; a.berr = berr; a.count = count;
+    a.queue = queue;
+    int grid = std::min(count, 2048);
+    hipLaunchKernelGGL(batch_residual_kernel, dim3(grid), dim3(256), 0, s, a);
+}
+
+/* X += R for every system with state[j] == 0 (`add`), or X = B for every
+ * system with state[j] != 1 (copy: the systems the FP64 fallback takes). */
+__global__ void batch_update_kernel(double *X, const double *R, const long long *boff, const int *np,
+                                    const int *nrhs, const int *state, int count, int add)
+{
+    const int j = blockIdx.x;
+    if (j >= count) return;
+    if (add ? state[j] != 0 : state[j] == 1) return;
+    const size_t len = (size_t)np[j] * nrhs[j];
+    double *x = X + boff[j];
+    const double *r = R + boff[j];
+    for (size_t e = threadIdx.x; e < len; e += blockDim.x) x[e] = add ? x[e] + r[e] : r[e];
+}
+
+void batch_update(double *X, const double *R, const long long *boff, const int *np, const int *nrhs,
+                  const int *state, int count, int add, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(batch_update_kernel, dim3(count), dim3(256), 0, s, X, R, boff, np, nrhs, state, count, add);
+}
+
+/* state[j] = (info[j] != 0) ? 2 : 0, iters[j] = 0, anrm[j] = -1 */
+__global__ void batch_refine_init_kernel(const int *info, int *state, int *iters, double *anrm, int count)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    state[j] = info[j] != 0 ? 2 : 0;
+    iters[j] = 0;
+    anrm[j] = -1.0;
+}
+
+void batch_refine_init(const int *info, int *state, int *iters, double *anrm, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(batch_refine_init_kernel, dim3((count + 255) / 256), dim3(256), 0, s, info, state, iters, anrm,
+                       count);
 }
 
 /* Clears n ints (queue counters) inside the stream order, without a

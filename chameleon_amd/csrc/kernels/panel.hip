@@ -221,7 +221,7 @@ struct PotrfJobs {
     int iinfo, batch;
 };
 
-template <int POTRF_WAVES>
+template <int POTRF_WAVES, bool LP = false>
 __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_kernel(PotrfJobs J)
 {
     constexpr int POTRF_THREADS = POTRF_WAVES * 64;
@@ -267,6 +267,9 @@ __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_
                 const bool active = rb < nrb;
                 const int r0 = C + rb * 16;
                 v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+                /* LP: the same products with FP32 operands and accumulation
+                 * (v_mfma_f32_16x16x4_f32, result slot s at column 4 lk + s) */
+                v4f accf[4] = {zero4f(), zero4f(), zero4f(), zero4f()};
                 const double *px = T + (r0 + li) + (size_t)lk * ldt;
                 /* Y slice element (row r < 64, k q < 16) -> Ys[q * 64 + r];
                  * thread tid moves rows 2*(tid%32)..+1 of columns
@@ -304,7 +307,13 @@ __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_
                         for (int q = 0; q < 4; ++q)
 #pragma unroll
                             for (int f = 0; f < 4; ++f)
-                                if (f < nfb) acc[f] = mfma_f64(Ys[cur][(4 * q + lk) * OB + 16 * f + li], x[q], acc[f]);
+                                if (f < nfb) {
+                                    if constexpr (LP)
+                                        accf[f] = mfma_f32((float)Ys[cur][(4 * q + lk) * OB + 16 * f + li], (float)x[q],
+                                                           accf[f]);
+                                    else
+                                        acc[f] = mfma_f64(Ys[cur][(4 * q + lk) * OB + 16 * f + li], x[q], acc[f]);
+                                }
                     }
                     if (more) {
 #pragma unroll
@@ -319,8 +328,9 @@ __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_
                     for (int f = 0; f < nfb; ++f)
 #pragma unroll
                         for (int s = 0; s < 4; ++s) {
-                            const int r = r0 + li, c = C + 16 * f + lk + 4 * s;
-                            if (r >= c) T[r + (size_t)c * ldt] -= acc[f][s];
+                            const int r = r0 + li;
+                            const int c = C + 16 * f + (LP ? 4 * lk + s : lk + 4 * s);
+                            if (r >= c) T[r + (size_t)c * ldt] -= LP ? (double)accf[f][s] : acc[f][s];
                         }
                 }
             }
@@ -456,26 +466,50 @@ void potrf_diag(double *Akk, int ldt, double *dinv, int iinfo, int nvalid, int *
  * zeroed device counter; jobs are taken in index order by `grid`
  * persistent workgroups (grid <= 0: as many as are resident at once; sort
  * the jobs largest first for balance). */
+template <int W, bool LP>
+static int potrf_jobs_resident()
+{
+    static int resident = 0;
+    if (!resident) {
+        int dev = 0, ncu = 0, per = 0;
+        CHX_HIP(hipGetDevice(&dev));
+        CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, potrf_jobs_kernel<W, LP>, W * 64, 0));
+        resident = ncu * (per > 0 ? per : 1);
+    }
+    return resident;
+}
+
+template <int W, bool LP>
+static void potrf_jobs_launch(const PotrfJobs &J, int grid, hipStream_t s)
+{
+    if (grid <= 0) grid = potrf_jobs_resident<W, LP>();
+    if (grid > J.count) grid = J.count;
+    hipLaunchKernelGGL((potrf_jobs_kernel<W, LP>), dim3(grid), dim3(W * 64), 0, s, J);
+}
+
 void potrf_jobs(double *A, const long long *off, const int *np, const int *ld, const int *nvalid, double *dinv,
-                const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s)
+                const long long *doff, int count, int *queue, int *info, int grid, hipStream_t s, int waves, bool lp)
 {
     PotrfJobs J = {};
     J.A = A; J.off = off; J.np = np; J.ld = ld; J.nvalid = nvalid; J.dinv = dinv; J.doff = doff;
     J.count = count; J.queue = queue; J.info = info; J.batch = 1;
     if (count <= 0) return;
-    if (grid <= 0) {
-        static int resident = 0;
-        if (!resident) {
-            int dev = 0, ncu = 0, per = 0;
-            CHX_HIP(hipGetDevice(&dev));
-            CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, potrf_jobs_kernel<4>, 256, 0));
-            resident = ncu * (per > 0 ? per : 1);
-        }
-        grid = resident;
+    if (waves == 8) {
+        if (lp) potrf_jobs_launch<8, true>(J, grid, s);
+        else potrf_jobs_launch<8, false>(J, grid, s);
+    } else {
+        if (lp) potrf_jobs_launch<4, true>(J, grid, s);
+        else potrf_jobs_launch<4, false>(J, grid, s);
     }
-    if (grid > count) grid = count;
-    hipLaunchKernelGGL(potrf_jobs_kernel<4>, dim3(grid), dim3(256), 0, s, J);
+}
+
+void potrf_jobs_prepare()
+{
+    (void)potrf_jobs_resident<4, false>();
+    (void)potrf_jobs_resident<8, false>();
+    (void)potrf_jobs_resident<4, true>();
+    (void)potrf_jobs_resident<8, true>();
 }
 
 /* ================================================================== */

@@ -198,6 +198,7 @@ struct StripArgs {
     int ldd;
     double *BT;           /* right-hand sides, element (c, r) at c + r * nrp         */
     int *flags;           /* one per strip and column chunk: chunk c at flags + c*S   */
+    unsigned *ticket;     /* task counter of the launch (cleared before it)           */
     int NT, ldt, nrp, c0; /* c0: first column of this launch's chunks                 */
     int nchunk;           /* column chunks of 16*NC columns solved side by side       */
     int S;                /* number of strips                                         */
@@ -246,9 +247,20 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
     const bool fwd = a.trans == 0;
     const bool lvec = !fwd;       /* backward reads L(j, s)^T as row vectors */
     /* task t = (strip in chain order) * nchunk + chunk: the chunks are
-     * independent chains, interleaved so that the tasks a workgroup waits
-     * for always have smaller indices */
-    for (int t = blockIdx.x; t < (a.S - a.s0) * a.nchunk; t += gridDim.x) {
+     * independent chains, interleaved so that the tasks a task waits for
+     * always have smaller indices.  Tasks are taken from a ticket counter
+     * (cleared before the launch), so every task a workgroup waits for
+     * belongs to a workgroup that has started: the chain moves whatever
+     * else occupies the device, no co-residency of the grid is needed. */
+    __shared__ int s_task;
+    unsigned *const ticket = a.ticket;
+    const int ntask = (a.S - a.s0) * a.nchunk;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_task = (int)atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int t = s_task;
+        if (t >= ntask) break;
         const int si = t / a.nchunk, cc = t - si * a.nchunk;
         const int s = fwd ? a.s0 + si : a.S - 1 - si;
         const int c0 = a.c0 + cc * NCOL;
@@ -431,17 +443,17 @@ static void strip_launch(const StripArgs &a, hipStream_t s)
     }
     int grid = ncu * per_cu;
     if (grid > (a.S - a.s0) * a.nchunk) grid = (a.S - a.s0) * a.nchunk;
-    StripArgs x = a;
-    void *args[] = {&x};
-    CHX_HIP(hipLaunchCooperativeKernel((const void *)strip_solve_kernel<NC>, dim3(grid), dim3(256), args, 0, s));
+    zero_ints(reinterpret_cast<int *>(a.ticket), 1, s);
+    hipLaunchKernelGGL(strip_solve_kernel<NC>, dim3(grid), dim3(256), 0, s, a);
 }
 
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
-                 int ldt, int nrp, int trans, int *flags, int epoch, const int *info, hipStream_t s, int s0)
+                 int ldt, int nrp, int trans, int *flags, int epoch, unsigned *ticket, const int *info, hipStream_t s,
+                 int s0)
 {
     StripArgs a;
     a.A = A; a.D = D; a.dts = dts; a.doff = doff; a.ldd = ldd;
-    a.BT = BT; a.flags = flags;
+    a.BT = BT; a.flags = flags; a.ticket = ticket;
     a.NT = NT; a.ldt = ldt; a.nrp = nrp;
     a.S = NT * ldt / SR;
     a.s0 = trans ? 0 : s0;

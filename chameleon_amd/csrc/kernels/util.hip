@@ -5,6 +5,8 @@
 #include "../internal.h"
 #include "rng.h"
 
+#include <algorithm>
+
 namespace chx {
 namespace k {
 
@@ -360,6 +362,44 @@ void lacpy_tiles(const double *src, double *dst, int mt, int nt, int ldt, int wd
     Geo g = geo(mt, nt, ldt, wdt, mb, nb, M, N);
     dim3 grid(16, mt * nt);
     hipLaunchKernelGGL(lacpy_kernel, grid, dim3(256), 0, s, src, dst, g, uplo);
+}
+
+} /* namespace k */
+} /* namespace chx */
+
+namespace chx {
+namespace k {
+
+/* ------------------------------------------------------------------ */
+/* Column panel <-> tile mirror.  P is a dense column-major block of
+ * `rows` x `cols` doubles (leading dimension rows) holding rows
+ * r0 .. r0+rows-1 and columns c0 .. c0+cols-1 of tile column n (global
+ * row g = r0 + i lies in tile row g / mb at row g % mb; column c0 + c of
+ * the tile column).  to_tiles = 1 scatters P into the tiles, 0 gathers the
+ * tiles into P.  One thread per element, consecutive threads down a
+ * column on both sides. */
+__global__ void panel_tiles_kernel(double *P, double *D, int mt, int ldt, long long tile_elems, int mb, int n, int r0,
+                                   int rows, int c0, int cols, int to_tiles)
+{
+    const long long total = (long long)rows * cols;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % rows), c = (int)(e / rows);
+        const int g = r0 + i, m = g / mb, ii = g - m * mb;
+        double *t = D + ((long long)n * mt + m) * tile_elems + ii + (long long)(c0 + c) * ldt;
+        if (to_tiles) *t = P[e];
+        else P[e] = *t;
+    }
+}
+
+void panel_tiles(double *P, double *D, int mt, int ldt, size_t tile_elems, int mb, int n, int r0, int rows, int c0,
+                 int cols, int to_tiles, hipStream_t s)
+{
+    const long long total = (long long)rows * cols;
+    if (total <= 0) return;
+    const long long blocks = std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(panel_tiles_kernel, dim3((unsigned)blocks), dim3(256), 0, s, P, D, mt, ldt,
+                       (long long)tile_elems, mb, n, r0, rows, c0, cols, to_tiles);
 }
 
 } /* namespace k */

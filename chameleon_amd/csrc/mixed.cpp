@@ -151,7 +151,7 @@ int backward_error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t
 {
     Context *c = ctx();
     hipStream_t s = c->s_main;
-    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, s);
+    mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, s, uplo == ChamLower);
     mirror_acquire(X, MIRROR_NORMAL, false, s);
     mirror_acquire(B, MIRROR_NORMAL, false, s);
     Mirror *MA = mirror_of(A), *MX = mirror_of(X), *MB = mirror_of(B);
@@ -182,7 +182,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
     Context *c = ctx();
     hipStream_t s = c->s_main;
     const int orient = uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL;
-    mirror_acquire(A, orient, true, s);
+    mirror_acquire(A, orient, true, s, uplo == ChamLower);
     mirror_acquire(B, MIRROR_NORMAL, false, s);
     mirror_prepare_overwrite(X, MIRROR_NORMAL, false);
     Mirror *MA = mirror_of(A), *MB = mirror_of(B), *MX = mirror_of(X);

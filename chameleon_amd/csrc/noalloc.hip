@@ -14,7 +14,9 @@
  *   [0, NOALLOC_FLAG_BYTES)       publication flags of the strip solves,
  *                                 tagged by launch epoch (never cleared
  *                                 between calls; cleared once at set-up,
- *                                 CHAMELEON_Init)
+ *                                 CHAMELEON_Init), and in its last int the
+ *                                 launches' task ticket (cleared before
+ *                                 each launch)
  *   [NOALLOC_FLAG_BYTES, ...)     doubles: the 64 x 64 inverses
  *                                 (Npad * 64), then the transposed
  *                                 right-hand sides (nrp * Npad)
@@ -119,6 +121,10 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
         error(fn, "A and B must be resident on the device (left there by an earlier _Tile_Async call)");
         return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
     }
+    if (MA->lower_only && want != MIRROR_NORMAL) {
+        error(fn, "A's device copy holds its lower tiles only (an Upper solve would need an upload)");
+        return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
+    }
     if (MA->ldt % 64 != 0 || MB->ldt != MA->ldt) {
         error(fn, "the padded tile size must be a multiple of 64");
         return request_fail(sequence, request, CHAMELEON_ERR_ILLEGAL_VALUE);
@@ -128,7 +134,7 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
     const int nrp = MB->nt * MB->wdt;
     const size_t ndinv = npad * 64, nbt = (size_t)nrp * npad;
     const int S = (int)(npad / 64);
-    if (ndinv + nbt > NOALLOC_MAX_DOUBLES || (size_t)4 * S * sizeof(int) > NOALLOC_FLAG_BYTES) {
+    if (ndinv + nbt > NOALLOC_MAX_DOUBLES || (size_t)(4 * S + 1) * sizeof(int) > NOALLOC_FLAG_BYTES) {
         error(fn, "problem larger than the fixed scratch array (N_padded * (64 + NRHS_padded) <= 4194304)");
         return request_fail(sequence, request, CHAMELEON_ERR_OUT_OF_RESOURCES);
     }
@@ -152,6 +158,9 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
     const int epoch = d->epoch + 1;
     d->epoch += 2 * chunks;
     int *flags = (int *)d->ptr;
+    /* the strip solves' task ticket: the last int of the flag region, past
+     * every flag (at most 4 * S <= 4096 of them) */
+    unsigned *ticket = (unsigned *)(d->ptr + NOALLOC_FLAG_BYTES) - 1;
     double *dinv = (double *)(d->ptr + NOALLOC_FLAG_BYTES);
     double *BT = dinv + ndinv;
     SeqState *st = seq_state(sequence);
@@ -180,7 +189,7 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
     for (int dir = 0; dir < 2; ++dir)
         k::solve_strips(Ad, dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
-                        st->d_info, s);
+                        ticket, st->d_info, s);
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 0, s);
     st->pending_info = true;
     mirror_mark_written(B);

@@ -89,8 +89,8 @@ int *lane_flags(Lane *L, int n, int launches, int *epoch)
 unsigned *lane_counters(Lane *L)
 {
     if (!L->panel_ctr) {
-        CHX_HIP(hipMalloc(&L->panel_ctr, 2 * sizeof(unsigned)));
-        k::zero_ints((int *)L->panel_ctr, 2, L->sm);
+        CHX_HIP(hipMalloc(&L->panel_ctr, 4 * sizeof(unsigned)));
+        k::zero_ints((int *)L->panel_ctr, 4, L->sm);
     }
     return L->panel_ctr;
 }
@@ -161,9 +161,14 @@ static void mirror_alloc(CHAM_desc_t *A, Mirror *M)
     M->state = MIRROR_NONE;
 }
 
-/* host -> device, normal orientation */
-static void mirror_upload(CHAM_desc_t *A, Mirror *M, hipStream_t s)
+/* host -> device, normal orientation; column-major double matrices by
+ * column panels through pinned staging (transfer.cpp), `part` of them */
+static void mirror_upload(CHAM_desc_t *A, Mirror *M, hipStream_t s, int part = XFER_ALL)
 {
+    if (xfer_eligible(A, M)) {
+        xfer_upload(A, M, part, s);
+        return;
+    }
     const int es = M->eltsize;
     for (int n = 0; n < A->nt; ++n) {
         const int tn = A->get_blkdim(A, n, DIM_n, A->n);
@@ -181,6 +186,10 @@ static void mirror_upload(CHAM_desc_t *A, Mirror *M, hipStream_t s)
 
 static void mirror_download(CHAM_desc_t *A, Mirror *M, hipStream_t s)
 {
+    if (xfer_eligible(A, M)) {
+        xfer_download(A, M, M->lower_only ? XFER_LOWER : XFER_ALL, s);
+        return;
+    }
     const int es = M->eltsize;
     for (int n = 0; n < A->nt; ++n) {
         const int tn = A->get_blkdim(A, n, DIM_n, A->n);
@@ -207,16 +216,31 @@ static void mirror_orient(Mirror *M, int state, hipStream_t s)
     M->state = state;
 }
 
-int mirror_acquire(CHAM_desc_t *A, int state, bool diag_identity, hipStream_t s)
+void mirror_complete(CHAM_desc_t *A, hipStream_t s)
+{
+    Mirror *M = mirror_of(A);
+    if (!M || !M->lower_only) return;
+    /* the upper tiles were never written on the device: the host holds them */
+    xfer_upload(A, M, XFER_UPPER, s);
+    k::fix_padding(M->d, M->mt, M->nt, M->ldt, M->wdt, A->mb, A->nb, A->m, A->n, M->eltsize, M->diag_identity, s);
+    M->lower_only = false;
+}
+
+int mirror_acquire(CHAM_desc_t *A, int state, bool diag_identity, hipStream_t s, bool lower_need)
 {
     Mirror *M = mirror_get(A);
     mirror_alloc(A, M);
     if (M->state == MIRROR_NONE) {
-        mirror_upload(A, M, s);
+        const bool part = lower_need && state == MIRROR_NORMAL && A->m == A->n && xfer_eligible(A, M);
+        mirror_upload(A, M, s, part ? XFER_LOWER : XFER_ALL);
+        M->lower_only = part;
         k::fix_padding(M->d, M->mt, M->nt, M->ldt, M->wdt, A->mb, A->nb, A->m, A->n, M->eltsize, diag_identity, s);
         M->diag_identity = diag_identity;
         M->state = MIRROR_NORMAL;
-    } else if (M->diag_identity != diag_identity) {
+    } else if (M->lower_only && !(lower_need && state == MIRROR_NORMAL)) {
+        mirror_complete(A, s);
+    }
+    if (M->diag_identity != diag_identity) {
         /* padding does not depend on orientation for square tile matrices */
         k::fix_padding(M->d, M->mt, M->nt, M->ldt, M->wdt, A->mb, A->nb, A->m, A->n, M->eltsize, diag_identity, s);
         M->diag_identity = diag_identity;
@@ -229,6 +253,7 @@ int mirror_prepare_overwrite(CHAM_desc_t *A, int state, bool diag_identity)
 {
     Mirror *M = mirror_get(A);
     mirror_alloc(A, M);
+    M->lower_only = false;
     M->state = state;
     M->diag_identity = diag_identity;
     return CHAMELEON_SUCCESS;
@@ -400,6 +425,7 @@ int __chameleon_finalize(void)
     }
     CHX_HIP(hipDeviceSynchronize());
     latms_release();
+    xfer_release();
     lane_release(&c->lane);
     stats_fold();
     for (hipEvent_t e : c->timing_pool) CHX_HIP(hipEventDestroy(e));

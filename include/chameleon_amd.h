@@ -162,13 +162,46 @@ int CHAMAMD_dpotrs_batch_Async(CHAMAMD_batch_t *batch, RUNTIME_sequence_t *seque
 int CHAMAMD_dposv_batch_Async(CHAMAMD_batch_t *batch, RUNTIME_sequence_t *sequence, RUNTIME_request_t *request);
 /* per-system info of the last factorisation (after the sequence completed) */
 int CHAMAMD_Batch_Info(CHAMAMD_batch_t *batch, int *info);
+/* Mixed-precision solve of every system with FP64 refinement (LAPACK
+ * dsposv semantics per system; prec = CHAMAMD_PREC_FP32: the factorisation
+ * of a working copy multiplies FP32 operands with FP32 accumulation in its
+ * left-looking updates, the 64 x 64 diagonal blocks stay FP64).  Residuals
+ * and the stopping test ||r|| <= ||x|| ||A|| eps sqrt(n) run per system on
+ * the device, at most 30 corrections; systems whose reduced-precision
+ * factorisation failed or did not converge are factored and solved in FP64.
+ * B receives the solutions; iter[i] = refinement steps, -3 or -31 (may be
+ * NULL); info[i] = the FP64 fallback's info (0 otherwise).  A keeps its
+ * values for converged systems and holds the FP64 factor for the others.
+ * Returns the first nonzero info in system order, or 0. */
+int CHAMAMD_dposv_mixed_batch(CHAMAMD_batch_t *batch, int prec, int *iter, int *info);
+/* Size buckets of the batched factorisation, by padded order: <= 64, 128,
+ * 256, 512, 1024, larger (6 buckets).  Each bucket is factored with the
+ * workgroup shape -- 4 or 8 wavefronts per system -- of a process-wide
+ * table.  CHAMAMD_Batch_Tune times both shapes on a sample of every
+ * bucket `batch` holds (bumped dplgsy matrices of the same orders, in
+ * scratch memory: the batch's matrices are untouched), keeps the faster
+ * in the table and copies the table to waves[6] (may be NULL).  It runs by
+ * itself on a batch's first factorisation while CHAMELEON_AUTOTUNING is
+ * enabled (setting CHAMELEON_TILE_SIZE disables it, as in the reference).
+ * Get/Set read and write the table (and the buckets' upper bounds, and
+ * whether each was tuned in this process). */
+int CHAMAMD_Batch_Tune(CHAMAMD_batch_t *batch, int *waves);
+int CHAMAMD_Batch_Get_Tuning(int *waves, int *bucket_top, int *tuned);
+int CHAMAMD_Batch_Set_Tuning(const int *waves);
+/* on = 1 (the default, or CHAMAMD_BATCH_GRAPHS): a batch's factorisation
+ * launches and its solve launch are captured into HIP graphs on first use
+ * and replayed; 0: issued directly every time.  Results are the same. */
+int CHAMAMD_Set_Batch_Graphs(int on);
 
 /* ------------------------------------------------------------------ */
-/* Solve server: requests A X = B arrive over time; factors stay resident
- * in a device pool of a fixed HBM budget (least recently used factors not
- * needed by pending requests are evicted), and each Progress call serves
- * the oldest pending requests as one batched round.  No counterpart in the
- * reference beyond its sequence/request interface
+/* Solve server with continuous batching: requests A X = B arrive over time;
+ * factors stay resident in a device pool of a fixed HBM budget (least
+ * recently used factors no live request needs are evicted); work is
+ * admitted onto free lanes (CHAMAMD_SERVE_LANES streams, default 4) as they
+ * and memory free up, while earlier work runs -- new small systems
+ * factored together, large ones one per lane, ready small requests solved
+ * in one batched launch, large ones by the strip solve.  No counterpart in
+ * the reference beyond its sequence/request interface
  * (include/chameleon/runtime_struct.h:78-103), which has no admission
  * control, residency or latency accounting.  Host-thread model as the
  * rest of the API: one thread calls in. */
@@ -186,18 +219,41 @@ typedef struct {
     long long completed, rejected, factored, reused, resident, pool_used_bytes;
     double p50_ms, p99_ms, max_ms;   /* submit-to-solution latency of completed requests      */
     double flops;                    /* factor + solve flops served (Chameleon's flop model)   */
+    long long mixed_solves;          /* large requests solved on a mixed-precision factor      */
+    long long ir_steps;              /* refinement corrections over those solves               */
+    long long fallbacks;             /* mixed factorisations or refinements redone in FP64     */
+    long long max_jobs;              /* most lanes busy at once                                */
+    double max_berr;                 /* largest backward error ||r||/(||A|| ||x|| N eps) of a
+                                        refined solution                                       */
 } CHAMAMD_server_stats_t;
 
 int CHAMAMD_Server_Create(CHAMAMD_server_t **server, size_t hbm_budget_bytes);
 int CHAMAMD_Server_Destroy(CHAMAMD_server_t **server);
+/* Products of the factorisations of large systems (order > 1024) made
+ * after the call: -1 (the default) FP64; CHAMAMD_PREC_FP32, _BF16,
+ * _BF16X3 or _FP8 low-precision trailing products with FP64 iterative
+ * refinement of every solve on the FP64 matrix kept beside the factor
+ * (LAPACK dsposv's stopping test; a matrix whose factorisation or
+ * refinement fails is factored again in FP64).  Small systems are
+ * factored in FP64.  Returns CHAMELEON_SUCCESS or
+ * CHAMELEON_ERR_ILLEGAL_VALUE. */
+int CHAMAMD_Server_Set_Precision(CHAMAMD_server_t *server, int prec);
 int CHAMAMD_Server_Submit(CHAMAMD_server_t *server, const CHAMAMD_request_t *req, long long *ticket);
-/* serve one round of the oldest pending requests (blocking; at most
- * CHAMAMD_SERVE_ROUND of them, default 256); *processed = requests taken */
+/* Non-blocking: collect finished device work, admit pending requests into
+ * free memory and lanes, start what can start; *processed = requests
+ * finished during the call.  B of a finished request holds its solution. */
 int CHAMAMD_Server_Progress(CHAMAMD_server_t *server, int *processed);
+/* Progress until every submitted request has finished (waiting on the
+ * device in between); requests that can never be placed finish with
+ * CHAMELEON_ERR_OUT_OF_RESOURCES. */
+int CHAMAMD_Server_Drain(CHAMAMD_server_t *server);
 /* *done = 1 once the request finished; *status = 0, the not-positive-
  * definite column, or a negative error (CHAMELEON_ERR_OUT_OF_RESOURCES
  * when its factor cannot fit the budget) */
 int CHAMAMD_Server_Test(CHAMAMD_server_t *server, long long ticket, int *done, int *status);
+/* tickets of requests finished since the last call, oldest first: up to
+ * `cap` of them into tickets[0 .. *count-1] (the rest on the next call) */
+int CHAMAMD_Server_Finished(CHAMAMD_server_t *server, long long *tickets, int cap, int *count);
 int CHAMAMD_Server_Stats(CHAMAMD_server_t *server, CHAMAMD_server_stats_t *stats);
 int CHAMAMD_Server_Reset_Stats(CHAMAMD_server_t *server);
 

@@ -252,3 +252,173 @@ def test_uneven_batch_async_and_empty(ch):
         assert oracle.rel_frobenius(b.get_b(i), X) <= 1e-11
     with Batch(ch, [], []) as e:
         assert e.posv()[0] == 0
+
+
+def _batch_sizes(count, seed):
+    rng = np.random.default_rng(seed)
+    n = np.floor(np.exp(rng.uniform(np.log(16), np.log(1300), count))).astype(np.int32)
+    n[:3] = [1100, 1300, 17]            # the bucket above 1024 and the smallest one
+    nrhs = rng.integers(1, 9, count).astype(np.int32)
+    return n, nrhs
+
+
+def _solve_all(ch, b, seed):
+    ch.check(ch.lib.CHAMAMD_dplgsy_batch(b.ptr, seed))
+    ch.check(ch.lib.CHAMAMD_dplrnt_batch(b.ptr, seed))
+    rc, info = b.posv()
+    assert rc == 0 and not info.any()
+    return [b.get_b(i).copy() for i in range(len(b.n))], [b.get_a(i) for i in (0, 1, 5)]
+
+
+def test_batch_graph_replays_equal_eager(ch):
+    """The factorisation and solve launches replayed from their captured
+    graphs, five times with eager work (regeneration, reads) in between,
+    give bitwise the results of launching them directly."""
+    n, nrhs = _batch_sizes(400, 21)
+    say("batch graphs", len(n))
+    with Batch(ch, n, nrhs) as b:
+        try:
+            assert ch.lib.CHAMAMD_Set_Batch_Graphs(0) == 0
+            X0, F0 = _solve_all(ch, b, 77)
+            assert ch.lib.CHAMAMD_Set_Batch_Graphs(1) == 0
+            for rep in range(5):
+                X, F = _solve_all(ch, b, 77)
+                for i in range(len(n)):
+                    assert np.array_equal(X[i], X0[i]), (rep, i)
+                for f, f0 in zip(F, F0):
+                    assert np.array_equal(f, f0), rep
+        finally:
+            ch.lib.CHAMAMD_Set_Batch_Graphs(1)
+    for i in (0, 1, 2, 9):
+        ni, ri = int(n[i]), int(nrhs[i])
+        A0 = oracle.dplgsy(float(ni), UL_, ni, 77 + i)
+        assert oracle.check_solve(A0, X0[i], oracle.dplrnt(ni, ri, 77 + i))[0], i
+
+
+def test_batch_tuner_and_shapes(ch):
+    """CHAMAMD_Batch_Tune picks 4 or 8 wavefronts for every bucket the batch
+    holds; both shapes compute every system in the same order of
+    operations, so a batch factored with either gives bitwise the same
+    factors and solutions, equal to the oracle's."""
+    n, nrhs = _batch_sizes(300, 22)
+    say("batch tuner", len(n))
+    tops = (ctypes.c_int * 6)()
+    before = (ctypes.c_int * 6)()
+    ch.check(ch.lib.CHAMAMD_Batch_Get_Tuning(before, tops, None))
+    assert list(tops)[:5] == [64, 128, 256, 512, 1024]
+    try:
+        with Batch(ch, n, nrhs) as b:
+            table = (ctypes.c_int * 6)()
+            ch.check(ch.lib.CHAMAMD_Batch_Tune(b.ptr, table))
+            tuned = (ctypes.c_int * 6)()
+            ch.check(ch.lib.CHAMAMD_Batch_Get_Tuning(None, None, tuned))
+            assert all(w in (4, 8) for w in table) and all(tuned), (list(table), list(tuned))
+            res = {}
+            for w in (4, 8):
+                ch.check(ch.lib.CHAMAMD_Batch_Set_Tuning((ctypes.c_int * 6)(*([w] * 6))))
+                res[w] = _solve_all(ch, b, 55)
+            ch.check(ch.lib.CHAMAMD_Batch_Set_Tuning((ctypes.c_int * 6)(4, 8, 4, 8, 4, 8)))
+            res["mix"] = _solve_all(ch, b, 55)
+            for key in (8, "mix"):
+                for i in range(len(n)):
+                    assert np.array_equal(res[key][0][i], res[4][0][i]), (key, i)
+                for f, f0 in zip(res[key][1], res[4][1]):
+                    assert np.array_equal(f, f0), key
+            assert ch.lib.CHAMAMD_Batch_Set_Tuning((ctypes.c_int * 6)(*([5] * 6))) == ch.ERR_ILLEGAL_VALUE
+    finally:
+        ch.lib.CHAMAMD_Batch_Set_Tuning(before)
+    oracle.use_blas(True)
+    try:
+        for i in (0, 1, 2, 40):
+            ni, ri = int(n[i]), int(nrhs[i])
+            Ar = oracle.dplgsy(float(ni), UL_, ni, 55 + i)
+            Br = oracle.dplrnt(ni, ri, 55 + i)
+            assert oracle.dposv(L_, Ar, ni, Br, ri, 64) == 0
+            assert oracle.rel_frobenius(res[4][0][i], Br) <= 1e-11, i
+    finally:
+        oracle.use_blas(False)
+
+
+FP32 = 0
+
+
+def _mixed(ch, b):
+    count = len(b.n)
+    it = np.zeros(count, dtype=np.int32)
+    info = np.zeros(count, dtype=np.int32)
+    rc = ch.lib.CHAMAMD_dposv_mixed_batch(b.ptr, FP32, ip(it), ip(info))
+    return rc, it, info
+
+
+def test_batch_mixed_fp32_refined(ch):
+    """FP32-product factorisations refined in FP64 per system: every system
+    converges (bumped generator, condition number ~1), its solution is the
+    FP64 one (oracle) and passes Chameleon's check, A is left unchanged,
+    and a system refined alone is bitwise the same as in the batch."""
+    n, nrhs = _batch_sizes(300, 23)
+    say("batch mixed fp32", len(n))
+    with Batch(ch, n, nrhs) as b:
+        ch.check(ch.lib.CHAMAMD_dplgsy_batch(b.ptr, 91))
+        ch.check(ch.lib.CHAMAMD_dplrnt_batch(b.ptr, 91))
+        A_before = [b.get_a(i) for i in (0, 1, 7)]
+        rc, it, info = _mixed(ch, b)
+        assert rc == 0 and not info.any()
+        assert it.min() >= 0 and it.max() <= 30, (it.min(), it.max())
+        X = [b.get_b(i).copy() for i in range(len(n))]
+        for k, i in enumerate((0, 1, 7)):
+            assert np.array_equal(b.get_a(i), A_before[k]), i
+    oracle.use_blas(True)
+    try:
+        for i in (0, 1, 2, 7, 50, 299):
+            ni, ri = int(n[i]), int(nrhs[i])
+            Ar = oracle.dplgsy(float(ni), UL_, ni, 91 + i)
+            Br = oracle.dplrnt(ni, ri, 91 + i)
+            B0 = Br.copy(order="F")
+            assert oracle.dposv(L_, Ar.copy(order="F"), ni, Br, ri, 64) == 0
+            assert oracle.rel_frobenius(X[i], Br) <= 1e-11, i
+            assert oracle.check_solve(Ar, X[i], B0)[0], i
+    finally:
+        oracle.use_blas(False)
+    # alone == in the batch
+    for i in (1, 7):
+        ni, ri = int(n[i]), int(nrhs[i])
+        with Batch(ch, [ni], [ri]) as one:
+            one.set_a(0, oracle.dplgsy(float(ni), L_, ni, 91 + i))
+            one.set_b(0, np.asfortranarray(oracle.dplrnt(ni, ri, 91 + i)))
+            rc, it1, info1 = _mixed(ch, one)
+            assert rc == 0 and it1[0] == it[i]
+            assert np.array_equal(one.get_b(0), X[i]), i
+
+
+def test_batch_mixed_fallbacks(ch):
+    """A non-SPD system stops its reduced-precision factorisation (ITER -3)
+    and reports LAPACK's info from the FP64 fallback; a system of condition
+    number 1e12 cannot be solved from FP32 products (ITER -3 or -31) and is
+    solved in FP64; the systems beside them are not disturbed."""
+    say("batch mixed fallbacks")
+    sizes = [300, 200, 150, 90]
+    with Batch(ch, sizes, [2, 1, 3, 1]) as b:
+        rng = np.random.default_rng(4)
+        mats = []
+        for i, ni in enumerate(sizes):
+            if i == 0:
+                Q, _ = np.linalg.qr(rng.standard_normal((ni, ni)))
+                lam = 1e12 ** (-np.arange(ni) / (ni - 1))
+                A = (Q * lam) @ Q.T
+                A = np.asfortranarray((A + A.T) / 2)
+            elif i == 2:
+                A = np.asfortranarray(np.eye(ni))
+                A[100, 100] = -2.0
+            else:
+                A = oracle.dplgsy(float(ni), UL_, ni, 40 + i)
+            mats.append(A)
+            b.set_a(i, A, UL_)
+            b.set_b(i, np.asfortranarray(rng.uniform(-0.5, 0.5, (ni, int(b.nrhs[i])))))
+        B0 = [b.get_b(i).copy() for i in range(len(sizes))]
+        rc, it, info = _mixed(ch, b)
+        assert rc == 101 and list(info) == [0, 0, 101, 0], (rc, list(info))
+        # condition 1e12: the FP32-product factorisation either meets a
+        # non-positive pivot (-3) or cannot be refined (-31)
+        assert it[0] in (-3, -31) and it[2] == -3 and it[1] >= 0 and it[3] >= 0, list(it)
+        for i in (0, 1, 3):
+            assert oracle.check_solve(mats[i], b.get_b(i), B0[i])[0], i
