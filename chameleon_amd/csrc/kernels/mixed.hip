@@ -338,124 +338,6 @@ __global__ __launch_bounds__(512, 4) void update_bf16_kernel(double *A, const un
             }
 }
 
-/* Pipelined form of the BF16-operand update: 32-deep k slices, two LDS
- * stages (stage s+1 is stored while stage s is multiplied) and global
- * loads issued two slices ahead, one barrier per slice.  LDS rows are the
- * 32 k of one row (64 bytes), so the 16 rows x 4 k-groups a wavefront reads
- * per operand are one contiguous 1 KiB block. */
-template <int SPLIT>
-__global__ __launch_bounds__(512, 2) void update_bf16p_kernel(double *A, const unsigned short *P,
-                                                              const unsigned short *P2, int NT, int lmt, int ldt,
-                                                              int n0, int ncol, int total_blocks, const int *info,
-                                                              const unsigned char *pmap)
-{
-    constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 32;
-    constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
-    constexpr int NP = (SPLIT == 3) ? 2 : 1;          /* planes per operand */
-    __shared__ unsigned short As[2][NP][BM * BK];
-    __shared__ unsigned short Bs[2][NP][BN * BK];
-    if (*info != 0) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int bid = xcd_remap_m(blockIdx.x, total_blocks);
-    const int tbm = ldt / BM, tbn = ldt / BN, bpt = tbm * tbn;
-    int m, n;
-    decode_pair(bid / bpt, n0, ncol, NT, m, n);
-    if (pmap && pmap[m + (size_t)n * NT] != 2) return;
-    const int b = bid % bpt, row0 = (b % tbm) * BM, col0 = (b / tbm) * BN;
-    const bool diag = (m == n);
-    if (diag && col0 >= row0 + BM) return;
-    const size_t ts = (size_t)ldt * ldt;
-    double *C = A + ((size_t)n * lmt + m) * ts;
-    const int wm = (wave % (BM / WM)) * WM, wn = (wave / (BM / WM)) * WN;
-    v4f acc[FM][FN];
-#pragma unroll
-    for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = zero4f();
-    /* one 16-byte piece of each operand plane per thread and slice:
-     * row tid / 4, k 8 * (tid % 4) */
-    const int lr = tid >> 2, lq = (tid & 3) * 8;
-    const unsigned short *pa[NP], *pb[NP];
-#pragma unroll
-    for (int pl = 0; pl < NP; ++pl) {
-        pa[pl] = P + ((size_t)pl * NT + m) * ts + (size_t)(row0 + lr) * ldt + lq;
-        pb[pl] = P + ((size_t)pl * NT + n) * ts + (size_t)(col0 + lr) * ldt + lq;
-    }
-    v8u16 ra[NP], rb[NP];
-    /* slices of the first panel, then (P2 given) of the second */
-    const int SK = ldt / BK;
-    const ptrdiff_t d2 = P2 ? P2 - P : 0;
-    auto gload = [&](int st) {
-        const int pi = st / SK, kb = (st - pi * SK) * BK;
-#pragma unroll
-        for (int pl = 0; pl < NP; ++pl) {
-            ra[pl] = *reinterpret_cast<const v8u16 *>(pa[pl] + pi * d2 + kb);
-            rb[pl] = *reinterpret_cast<const v8u16 *>(pb[pl] + pi * d2 + kb);
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int pl = 0; pl < NP; ++pl) {
-            *reinterpret_cast<v8u16 *>(&As[buf][pl][lr * BK + lq]) = ra[pl];
-            *reinterpret_cast<v8u16 *>(&Bs[buf][pl][lr * BK + lq]) = rb[pl];
-        }
-    };
-    const int S = (P2 ? 2 : 1) * SK;
-    gload(0);
-    lstore(0);
-    if (S > 1) gload(1);
-    __syncthreads();
-    for (int st = 0; st < S; ++st) {
-        const int cur = st & 1;
-        if (st + 1 < S) {
-            lstore(cur ^ 1);
-            if (st + 2 < S) gload(st + 2);
-        }
-        v8bf ah[FM], bh[FN], al[FM], bl[FN];
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm) {
-            ah[fm] = *reinterpret_cast<const v8bf *>(&As[cur][0][(wm + fm * 16 + li) * BK + 8 * lk]);
-            if (SPLIT == 3) al[fm] = *reinterpret_cast<const v8bf *>(&As[cur][NP - 1][(wm + fm * 16 + li) * BK + 8 * lk]);
-        }
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn) {
-            bh[fn] = *reinterpret_cast<const v8bf *>(&Bs[cur][0][(wn + fn * 16 + li) * BK + 8 * lk]);
-            if (SPLIT == 3) bl[fn] = *reinterpret_cast<const v8bf *>(&Bs[cur][NP - 1][(wn + fn * 16 + li) * BK + 8 * lk]);
-        }
-        /* the three products of one accumulator are FM*FN products apart,
-         * so no MFMA waits for the result of the one just before it */
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
-                acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[fn], ah[fm], acc[fm][fn], 0, 0, 0);
-        if (SPLIT == 3) {
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
-                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[fn], ah[fm], acc[fm][fn], 0, 0, 0);
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-                for (int fn = 0; fn < FN; ++fn)
-                    acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[fn], al[fm], acc[fm][fn], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn)
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int gi = row0 + wm + fm * 16 + li;
-                const int gj = col0 + wn + fn * 16 + 4 * lk + s;
-                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= (double)acc[fm][fn][s];
-            }
-}
-
 /* ------------------------------------------------------------------ */
 /* FP8 (OCP E4M3: 4 exponent bits, bias 7, 3 mantissa bits, largest finite
  * 448, subnormals from 2^-9) operands with one power-of-two scale per
@@ -761,12 +643,7 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
         hipLaunchKernelGGL(update_fp8_kernel, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned char *)P,
                            (const unsigned char *)P2, NT, lmt, ldt, n0, ncol, (int)total, info);
     else {
-        static const int pipelined = [] {
-            const char *e = getenv("CHAMAMD_LP_PIPE");
-            return e ? atoi(e) : 0;
-        }();
-        auto *kern = pipelined ? (mode == 1 ? update_bf16p_kernel<1> : update_bf16p_kernel<3>)
-                               : (mode == 1 ? update_bf16_kernel<1> : update_bf16_kernel<3>);
+        auto *kern = mode == 1 ? update_bf16_kernel<1> : update_bf16_kernel<3>;
         hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
                            (const unsigned short *)P2, NT, lmt, ldt, n0, ncol, (int)total, info, pmap);
     }
