@@ -654,6 +654,9 @@ def main():
                                                  tuple(float(x) for x in args.kappas.split(",")),
                                                  tuple(int(x) for x in args.mixed_nrhs.split(",")),
                                                  tuple(args.mixed_precs.split(",")))
+    # the conditioned generator keeps its Q factor for the next call: free
+    # it before the batch and serving legs, which size their pools to HBM
+    ch.check(ch.lib.CHAMAMD_Cond_Cache_Release())
     batch = None if args.no_batch else batch_leg(ch)
     serve = None
     if args.serve > 0:

@@ -152,6 +152,7 @@ void potrf_run(const PotrfPlan &p)
     if (c->lookahead <= 0 || !su || NT <= 2) {
         for (int k = 0; k < NT; ++k) {
             panel(k);
+            xfer_stream_columns(p.A, k + 1, sm);
             update(k, k + 1, NT, sm);
         }
         return;
@@ -180,6 +181,7 @@ void potrf_run(const PotrfPlan &p)
             update(k, k + 4, NT, su, 2);
             CHX_HIP(hipEventRecord(event(2 * k + 1), su));
             if (p.after_panels) p.after_panels(p.after_arg, k + 1);
+            xfer_stream_columns(p.A, k + 2, sm);        /* tile columns 0..k+1 are final */
             if (last_bulk >= 0) CHX_HIP(hipStreamWaitEvent(sm, event(2 * last_bulk + 1), 0));
             last_bulk = k;
             update(k, k + 2, k + 4, sm, 2);
@@ -202,6 +204,7 @@ void potrf_run(const PotrfPlan &p)
         CHX_HIP(hipEventRecord(event(2 * k + 1), su));
         /* critical stream: column k+1, then panel k+1 */
         if (p.after_panels) p.after_panels(p.after_arg, k);
+        xfer_stream_columns(p.A, k + 1, sm);            /* tile columns 0..k are final */
         if (k >= 1) CHX_HIP(hipStreamWaitEvent(sm, event(2 * (k - 1) + 1), 0));
         update(k, k + 1, k + 2, sm);
         panel(k + 1);
@@ -349,6 +352,7 @@ int potrf_async(cham_uplo_t uplo, CHAM_desc_t *A, RUNTIME_sequence_t *sequence)
     if (A->m == 0) return CHAMELEON_SUCCESS;
     mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, c->s_main, uplo == ChamLower);
     Mirror *M = mirror_of(A);
+    if (c->stream_flush == A && uplo == ChamLower) xfer_stream_begin(A, M);
     SeqState *st = seq_state(sequence);
     PotrfPlan p;
     p.A = (double *)M->d;
@@ -493,6 +497,7 @@ int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequenc
     mirror_acquire(A, uplo == ChamUpper ? MIRROR_TRANS : MIRROR_NORMAL, true, sm, uplo == ChamLower);
     mirror_acquire(B, MIRROR_NORMAL, false, sm);
     Mirror *MA = mirror_of(A), *MB = mirror_of(B);
+    if (c->stream_flush == A && uplo == ChamLower) xfer_stream_begin(A, MA);
     if (!(potrf_solve_eligible(MA->ldt) && MB->ldt == MA->ldt)) {
         potrf_async(uplo, A, sequence);
         /* the solve kernels return at once when the device info flag is
@@ -582,8 +587,11 @@ int CHAMELEON_dposv_Tile_Async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B,
     return rc == CHAMELEON_SUCCESS ? rc : request_fail(sequence, request, rc);
 }
 
+/* stream_A: the call factors A (Lower, column-major storage): its factor
+ * columns are downloaded while the factorisation runs, the flush below
+ * only waits for the rest */
 static int sync_wrap(int (*fn)(RUNTIME_sequence_t *, RUNTIME_request_t *, void *), void *arg, CHAM_desc_t *A,
-                     CHAM_desc_t *B, const char *name)
+                     CHAM_desc_t *B, const char *name, bool stream_A = false)
 {
     if (!ctx()) {
         error(name, "CHAMELEON not initialized");
@@ -592,7 +600,9 @@ static int sync_wrap(int (*fn)(RUNTIME_sequence_t *, RUNTIME_request_t *, void *
     RUNTIME_sequence_t *sequence = nullptr;
     RUNTIME_request_t request = RUNTIME_REQUEST_INITIALIZER;
     CHAMELEON_Sequence_Create(&sequence);
+    if (stream_A && A) ctx()->stream_flush = A;
     fn(sequence, &request, arg);
+    ctx()->stream_flush = nullptr;
     if (A) CHAMELEON_Desc_Flush(A, sequence);
     if (B) CHAMELEON_Desc_Flush(B, sequence);
     int status = sequence_wait_internal(sequence);
@@ -608,7 +618,7 @@ int CHAMELEON_dposv_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B)
     return sync_wrap([](RUNTIME_sequence_t *s, RUNTIME_request_t *r, void *p) {
         PosvArgs *x = (PosvArgs *)p;
         return CHAMELEON_dposv_Tile_Async(x->uplo, x->A, x->B, s, r);
-    }, &a, A, B, "CHAMELEON_dposv_Tile");
+    }, &a, A, B, "CHAMELEON_dposv_Tile", uplo == ChamLower);
 }
 
 int CHAMELEON_dpotrf_Tile(cham_uplo_t uplo, CHAM_desc_t *A)
@@ -617,7 +627,7 @@ int CHAMELEON_dpotrf_Tile(cham_uplo_t uplo, CHAM_desc_t *A)
     return sync_wrap([](RUNTIME_sequence_t *s, RUNTIME_request_t *r, void *p) {
         PosvArgs *x = (PosvArgs *)p;
         return CHAMELEON_dpotrf_Tile_Async(x->uplo, x->A, s, r);
-    }, &a, A, nullptr, "CHAMELEON_dpotrf_Tile");
+    }, &a, A, nullptr, "CHAMELEON_dpotrf_Tile", uplo == ChamLower);
 }
 
 int CHAMELEON_dpotrs_Tile(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B)

@@ -118,6 +118,8 @@ struct Context {
     void  *pinned = nullptr;
     size_t pinned_bytes = 0;
     Lane lane;                        /* s_main / s_update and their events, flags, counters */
+    CHAM_desc_t *stream_flush = nullptr;  /* a synchronous Lower potrf/posv of this column-major
+                                             matrix downloads its factor while it runs */
     /* kernel statistics (CHAMELEON_GENERATE_STATS) */
     bool stats = false;
     struct Pending { int cls; hipEvent_t a, b; double flops; };
@@ -184,6 +186,16 @@ bool xfer_eligible(const CHAM_desc_t *A, const Mirror *M);
 void xfer_upload(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s);
 void xfer_download(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s);
 void xfer_release();
+/* Streaming flush of a Lower factorisation of a column-major descriptor:
+ * begin (after A's upload, lower tiles only), then the factorisation
+ * reports that its first ncols tile columns are final in the order of
+ * stream s (no-op unless the streamer serves that mirror), end waits for
+ * the download of every column (returns false when A is not streamed),
+ * drop abandons it (A null: whatever is streamed). */
+bool xfer_stream_begin(CHAM_desc_t *A, Mirror *M);
+void xfer_stream_columns(const double *mirror, int ncols, hipStream_t s);
+bool xfer_stream_end(CHAM_desc_t *A, hipStream_t s);
+void xfer_stream_drop(const CHAM_desc_t *A);
 int request_fail(RUNTIME_sequence_t *seq, RUNTIME_request_t *req, int status);
 
 /* sequences: device info slot for potrf of this sequence */

@@ -268,6 +268,11 @@ void mirror_mark_written(CHAM_desc_t *A)
 int mirror_flush(CHAM_desc_t *A, hipStream_t s)
 {
     Mirror *M = mirror_of(A);
+    if (M && xfer_stream_end(A, s)) {
+        /* the factor was downloaded column by column while it was formed */
+        M->host_valid = true;
+        return CHAMELEON_SUCCESS;
+    }
     if (!M || M->host_valid || M->state == MIRROR_NONE) return CHAMELEON_SUCCESS;
     if (A->mat == nullptr && A->use_mat) return CHAMELEON_ERR_UNALLOCATED;
     mirror_orient(M, MIRROR_NORMAL, s);
@@ -278,6 +283,7 @@ int mirror_flush(CHAM_desc_t *A, hipStream_t s)
 
 void mirror_release(CHAM_desc_t *A)
 {
+    xfer_stream_drop(A);
     Mirror *M = mirror_of(A);
     if (!M) return;
     if (M->d) {
@@ -425,6 +431,7 @@ int __chameleon_finalize(void)
     }
     CHX_HIP(hipDeviceSynchronize());
     latms_release();
+    xfer_stream_drop(nullptr);
     xfer_release();
     lane_release(&c->lane);
     stats_fold();

@@ -19,7 +19,7 @@
  * is rejected (CHAMELEON_ERR_OUT_OF_RESOURCES); one that fits only once
  * other requests finish waits for them.
  *
- * Lanes: CHAMAMD_SERVE_LANES (default 4) streams, each with its own panel
+ * Lanes: CHAMAMD_SERVE_LANES (default 8) streams, each with its own panel
  * counters, publication flags, info flag, device scratch and pinned host
  * staging.  A lane runs one job at a time; a job is
  *   - the factorisation of every new small matrix (order <= 1024) in one
@@ -765,9 +765,16 @@ void admit(Server *s)
         q->f = f;
         touch(s, f);
     }
+    /* new factors are placed in arrival order: after the first that does
+     * not fit, later requests only attach to factors already placed, so an
+     * overloaded server (thousands of requests waiting for memory) spends
+     * one failed placement per pass, not one per waiting request, and a
+     * large factor is not starved by smaller ones arriving after it */
+    bool blocked = false;
     for (Req *q : s->live) {
         if (q->f || q->state < 0) continue;
         auto it = s->factors.find(q->r.matrix_id);
+        if (it == s->factors.end() && blocked) continue;
         if (it != s->factors.end()) {          /* placed for an earlier request just now */
             Factor *f = it->second;
             if (f->n != q->r.n) {
@@ -790,6 +797,7 @@ void admit(Server *s)
         }
         if (!place(s, f)) {            /* fits once live requests release memory */
             delete f;
+            blocked = true;
             continue;
         }
         f->source = &q->r;
@@ -952,7 +960,7 @@ int CHAMAMD_Server_Create(CHAMAMD_server_t **server, size_t hbm_budget_bytes)
     s->pool.init(hbm_budget_bytes);
     const char *v = getenv("CHAMAMD_SERVE_SMALL");
     if (v && *v) s->small_max = std::max(16, atoi(v));
-    int lanes = 4;
+    int lanes = 8;
     v = getenv("CHAMAMD_SERVE_LANES");
     if (v && *v) lanes = std::min(16, std::max(1, atoi(v)));
     /* the strip solve needs tiles a multiple of 64 */

@@ -191,10 +191,12 @@ void xfer_upload(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s)
     }
 }
 
-void xfer_download(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s)
+namespace {
+
+/* gather, DMA into pinned staging, scatter into the user's array, the
+ * next chunks' gathers and DMAs issued before each scatter */
+void download_pieces(Staging *st, CHAM_desc_t *A, Mirror *M, const std::vector<Piece> &ps, hipStream_t s)
 {
-    Staging *st = staging();
-    const std::vector<Piece> ps = pieces(A, part);
     auto issue = [&](size_t p) {
         const Piece &pc = ps[p];
         const int k = (int)(p % XFER_CHUNKS);
@@ -220,6 +222,127 @@ void xfer_download(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s)
         });
         if (p + XFER_CHUNKS < ps.size()) issue(p + XFER_CHUNKS);
     }
+}
+
+} /* namespace */
+
+void xfer_download(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s)
+{
+    Staging *st = staging();
+    download_pieces(st, A, M, pieces(A, part), s);
+}
+
+/* ---------------------------------------------------------- streaming
+ * The factor of a Lower Cholesky is final tile column by tile column:
+ * column k no longer changes once panel k is done.  While a synchronous
+ * call factors a column-major matrix, a background thread downloads the
+ * final columns (their lower tiles) on a stream of its own as the
+ * factorisation reports them, so most of the flush happens during the
+ * factorisation; the flush then only waits for the rest. */
+namespace {
+
+struct Streamer {
+    CHAM_desc_t *A = nullptr;
+    Mirror *M = nullptr;
+    hipStream_t xs = nullptr;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::pair<int, hipEvent_t>> q;    /* tile columns final once the event fires */
+    bool stop = false;
+    int done = 0;                                 /* tile columns downloaded (worker)        */
+};
+
+Streamer *g_stream = nullptr;
+
+void stream_worker(Streamer *sr)
+{
+    Staging *st = staging();
+    for (;;) {
+        std::pair<int, hipEvent_t> item;
+        {
+            std::unique_lock<std::mutex> g(sr->mu);
+            sr->cv.wait(g, [&] { return sr->stop || !sr->q.empty(); });
+            if (sr->q.empty()) return;
+            item = sr->q.front();
+            sr->q.erase(sr->q.begin());
+        }
+        CHX_HIP(hipEventSynchronize(item.second));
+        CHX_HIP(hipEventDestroy(item.second));
+        const int upto = std::min(item.first, sr->A->nt);
+        if (upto <= sr->done) continue;
+        std::vector<Piece> ps;
+        for (const Piece &pc : pieces(sr->A, XFER_LOWER))
+            if (pc.n >= sr->done && pc.n < upto) ps.push_back(pc);
+        download_pieces(st, sr->A, sr->M, ps, sr->xs);
+        sr->done = upto;
+    }
+}
+
+} /* namespace */
+
+bool xfer_stream_begin(CHAM_desc_t *A, Mirror *M)
+{
+    if (g_stream || !xfer_eligible(A, M) || !M->lower_only || M->state != MIRROR_NORMAL) return false;
+    Staging *st = staging();
+    (void)st;
+    Streamer *sr = new Streamer();
+    sr->A = A;
+    sr->M = M;
+    CHX_HIP(hipStreamCreateWithFlags(&sr->xs, hipStreamNonBlocking));
+    sr->th = std::thread(stream_worker, sr);
+    g_stream = sr;
+    return true;
+}
+
+void xfer_stream_columns(const double *mirror, int ncols, hipStream_t s)
+{
+    Streamer *sr = g_stream;
+    if (!sr || sr->M->d != mirror) return;
+    hipEvent_t e;
+    CHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    CHX_HIP(hipEventRecord(e, s));
+    {
+        std::lock_guard<std::mutex> g(sr->mu);
+        sr->q.emplace_back(ncols, e);
+    }
+    sr->cv.notify_one();
+}
+
+bool xfer_stream_end(CHAM_desc_t *A, hipStream_t s)
+{
+    Streamer *sr = g_stream;
+    if (!sr || sr->A != A) return false;
+    /* the remaining columns once everything issued so far on s is done */
+    xfer_stream_columns((const double *)sr->M->d, A->nt, s);
+    {
+        std::lock_guard<std::mutex> g(sr->mu);
+        sr->stop = true;
+    }
+    sr->cv.notify_one();
+    sr->th.join();
+    CHX_HIP(hipStreamSynchronize(sr->xs));
+    CHX_HIP(hipStreamDestroy(sr->xs));
+    g_stream = nullptr;
+    delete sr;
+    return true;
+}
+
+void xfer_stream_drop(const CHAM_desc_t *A)
+{
+    Streamer *sr = g_stream;
+    if (!sr || (A && sr->A != A)) return;
+    {
+        std::lock_guard<std::mutex> g(sr->mu);
+        sr->stop = true;
+    }
+    sr->cv.notify_one();
+    sr->th.join();
+    for (auto &it : sr->q) CHX_HIP(hipEventDestroy(it.second));
+    CHX_HIP(hipStreamSynchronize(sr->xs));
+    CHX_HIP(hipStreamDestroy(sr->xs));
+    g_stream = nullptr;
+    delete sr;
 }
 
 void xfer_release()
