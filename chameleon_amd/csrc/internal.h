@@ -232,13 +232,16 @@ struct PotrfPlan {
      * of the same factor read them there) */
     Lane *lane = nullptr;
     double *dinv = nullptr;
+    /* potrf_solve_run with at most 8 right-hand sides: room for the 128 x 128
+     * strip inverses (NT*ldt*128 doubles), or null for the MFMA strip solve */
+    double *d128 = nullptr;
 };
 void potrf_run(const PotrfPlan &p);
 /* both strip solves of BT (nrp x NT*ldt, transposed right-hand sides) with
  * the Lower factor A and its 64 x 64 diagonal-block inverses, on lane L;
  * ldt a multiple of 64 */
 void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int ldt, double *BT, int nrp,
-                       const int *d_info);
+                       const int *d_info, const double *d128 = nullptr, int ncols = 0);
 /* potrf_run with the forward solve of BT (transposed right-hand sides,
  * nrp x NT*ldt, nrhs of them in use) issued beside the factorisation, then
  * the forward strips it left and the backward solve: BT <- A^-1 BT.  The
@@ -249,9 +252,13 @@ bool potrf_solve_eligible(int ldt);
 void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate);
 struct SolvePlan {
     double *TT, *TN, *BT, *W, *dinv;   /* diagonal-tile inverses, transposed RHS, one block, 64x64 inverses */
+    double *d128 = nullptr;            /* 128 x 128 strip inverses (narrow solves), or null */
+    int ncols = 0;                     /* right-hand-side columns in use */
     int R = 0;                         /* 64: strip solve; 0: tile-step solve */
 };
-SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info);
+/* ncols: columns of the right-hand sides in use (at most nrp); with few of
+ * them the plan also holds the 128 x 128 strip inverses */
+SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info, int ncols = 0);
 void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nrp, int *d_info);
 
 /* one system of a batched generation (kernels/batch.hip) */
@@ -370,9 +377,18 @@ void tile_gemm_nt(const double *A, int amt, const double *B, int bmt, double *C,
 /* solve.hip */
 /* ticket: one device counter, cleared by the call before each launch and
  * used by that launch only (launches on one stream may share it) */
+/* With D128 (dinv128 of the same factor) and 1 <= ncols <= 8 columns in
+ * use with ldt a multiple of 128, the narrow kernel solves the first ncols
+ * columns only (128-row strips, vector FMAs); otherwise the MFMA strip
+ * kernel solves all nrp columns. */
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
                  int ldt, int nrp, int trans, int *flags, int epoch, unsigned *ticket, const int *info, hipStream_t s,
-                 int s0 = 0);
+                 int s0 = 0, const double *D128 = nullptr, int ncols = 0);
+bool narrow_eligible(int ldt, int ncols);
+/* 128 x 128 inverses of the diagonal blocks of the 128-row strips (NT*ldt*128
+ * doubles at D128) from the 64 x 64 inverses D (addressed as in solve_strips) */
+void dinv128(const double *A, const double *D, long long dts, long long doff, int ldd, int NT, int ldt,
+             double *D128, hipStream_t s);
 void solve_coop(const double *A, const double *TN, const double *TT, double *BT, double *W, int NT, int ldt, int nrhs,
                 int trans, const int *info, hipStream_t s);
 /* with info != null, nothing is copied once *info is nonzero */

@@ -447,10 +447,353 @@ static void strip_launch(const StripArgs &a, hipStream_t s)
     hipLaunchKernelGGL(strip_solve_kernel<NC>, dim3(grid), dim3(256), 0, s, a);
 }
 
+/* ------------------------------------------------------------------ */
+/* Narrow right-hand sides (at most 8 columns): strips of 128 rows solved
+ * with FP64 FMAs on the vector ALUs.
+ *
+ * An FP64 MFMA computes 16 columns whether or not they are in use, and on
+ * this device the FP64 matrix and vector peaks are the same, so for one
+ * to eight columns the vector ALUs do the chain's arithmetic 2-16x sooner
+ * and strips twice as tall halve the number of chain links (each link is
+ * a few memory round trips: flag seen, solution block read, solution
+ * stored and flag set).  The strip's diagonal step uses the 128 x 128
+ * inverse of its diagonal block (D128, built by dinv128 from the 64 x 64
+ * inverses): X_s = D128_s acc (forward) or D128_s^T acc (backward).
+ *
+ * Workgroup of 512 threads per strip: thread (r, kq) = (tid % 128,
+ * tid / 128) owns row r of the strip and the k range [32 kq, 32 kq + 32)
+ * of every 128 x 128 factor block; it accumulates its partial products
+ * over all blocks in registers, and the four partial sums of a row meet
+ * once, in LDS, before the diagonal step.  Factor rows of the next block
+ * are loaded before the wait for the current one's publication (the
+ * factor does not depend on it); wave 0 alone polls the flags and reads
+ * the published block into LDS (double-buffered, one barrier per block).
+ * Tasks come from a ticket counter in chain order and publication is the
+ * same as in strip_solve_kernel (device-scope stores, store completion,
+ * barrier, flag = epoch), so the same argument shows every wait ends.   */
+#define NSR 128
+
+struct NarrowArgs {
+    const double *A;      /* factor mirror (Lower orientation)                          */
+    const double *D128;   /* 128 x 128 inverse of the diagonal block of strip s at
+                           * D128 + s * 16384, element (r, p) at r + p * 128            */
+    double *BT;           /* right-hand sides, element (c, r) at c + r * nrp            */
+    int *flags;           /* one per strip                                              */
+    unsigned *ticket;     /* task counter of the launch (cleared before it)              */
+    int NT, ldt, nrp;
+    int S, s0;            /* strips of 128 rows; forward: first strip solved            */
+    int trans, epoch;
+    const int *info;
+};
+
+template <int NCV, bool FWD>
+__global__ __launch_bounds__(512, 1) void strip_narrow_kernel(NarrowArgs a)
+{
+    __shared__ double Xs[2][NSR * NCV];   /* published blocks, element (p, c) at p*NCV + c */
+    __shared__ double P[4][NSR * NCV];    /* partial sums of the four k ranges               */
+    __shared__ double Acc[NSR * NCV];     /* input of the diagonal step                       */
+    __shared__ int s_task;
+    if (*a.info != 0) return;     /* uniform over the grid at launch */
+    /* wave and k range are uniform over a wavefront: scalar registers, so
+     * the uniform part of every load address is scalar too */
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), kq = wave >> 1;
+    const int r = tid & (NSR - 1);
+    const int ldt = a.ldt, nrp = a.nrp, q = ldt / NSR;
+    const size_t ts = (size_t)ldt * ldt;
+    constexpr bool fwd = FWD;
+    const int ntask = a.S - a.s0;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_task = (int)atomicAdd(a.ticket, 1u);
+        __syncthreads();
+        const int t = s_task;
+        if (t >= ntask) break;
+        const int s = fwd ? a.s0 + t : a.S - 1 - t;
+        const int tm = s / q, rs = (s % q) * NSR;
+        double *Bs = a.BT + (size_t)s * NSR * nrp;
+        auto block_of = [&](int j) -> const double * {
+            const int tj = j / q, rj = (j % q) * NSR;
+            return fwd ? a.A + ((size_t)tj * a.NT + tm) * ts + rs + (size_t)rj * ldt
+                       : a.A + ((size_t)tm * a.NT + tj) * ts + rj + (size_t)rs * ldt;
+        };
+        /* op(r, p) of block j for p in this thread's k range */
+        auto load_l = [&](int j, double (&l)[32]) {
+            const double *b = block_of(j);
+            if (fwd) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i) l[i] = b[r + (size_t)(32 * kq + i) * ldt];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const v4d v = *(const v4d *)(b + 32 * kq + 4 * i + (size_t)r * ldt);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) l[4 * i + e] = v[e];
+                }
+            }
+        };
+        double bval[NCV];
+        if (kq == 0) {
+#pragma unroll
+            for (int c = 0; c < NCV; ++c) bval[c] = Bs[c + (size_t)r * nrp];
+        }
+        /* row r (forward) or column r (backward) of the inverse, k range kq */
+        double dreg[32];
+        {
+            const double *Dg = a.D128 + (size_t)s * NSR * NSR;
+            if (fwd) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i) dreg[i] = Dg[r + (size_t)(32 * kq + i) * NSR];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const v4d v = *(const v4d *)(Dg + 32 * kq + 4 * i + (size_t)r * NSR);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) dreg[4 * i + e] = v[e];
+                }
+            }
+        }
+        double part[NCV];
+#pragma unroll
+        for (int c = 0; c < NCV; ++c) part[c] = 0.0;
+
+        int ready = fwd ? a.s0 - 1 : a.S;
+        /* wave 0 only: wait until strip j is published (bounded, as in
+         * strip_solve_kernel), then copy its solution block into Xs[buf] */
+        auto fetch = [&](int j, int buf) {
+            if (fwd ? j > ready : j < ready) {
+                for (int polls = 0;; ++polls) {
+                    const int jj = fwd ? j + lane : j - lane;
+                    const bool inr = fwd ? jj < s : jj > s;
+                    const bool set = inr && __hip_atomic_load(&a.flags[jj], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+                    const unsigned long long m = __ballot(set);
+                    if (m & 1ull) {
+                        const int run = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
+                        ready = fwd ? j + run - 1 : j - run + 1;
+                        break;
+                    }
+                    if ((polls & 63) == 63) {
+                        const int inf = __hip_atomic_load(a.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (inf == 0 && polls >= STRIP_SPIN_LIMIT && lane == 0)
+                            atomicCAS((int *)a.info, 0, PANEL_STALLED);
+                        if (inf != 0 || polls >= STRIP_SPIN_LIMIT) {
+                            ready = fwd ? a.S : -1;
+                            break;
+                        }
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            }
+            const double *Xj = a.BT + (size_t)j * NSR * nrp;
+            double xv[2][NCV];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) {
+                    const unsigned long long u = __hip_atomic_load(
+                        (const unsigned long long *)&Xj[c + (size_t)(lane + 64 * h) * nrp], __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_AGENT);
+                    xv[h][c] = __longlong_as_double((long long)u);
+                }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) Xs[buf][(lane + 64 * h) * NCV + c] = xv[h][c];
+        };
+
+        /* block j's factor rows are requested before the wait for its
+         * solution, so their latency hides behind the wait; one block in
+         * flight per workgroup (128 KB) is enough to keep HBM busy */
+        const int nblk = fwd ? s - a.s0 : a.S - 1 - s;
+        int j = fwd ? a.s0 : a.S - 1;
+        for (int b = 0; b < nblk; ++b) {
+            double l[32];
+            load_l(j, l);
+            /* Xs[b&1] was last read in iteration b-2, before the barrier of
+             * iteration b-1 */
+            if (wave == 0) fetch(j, b & 1);
+            __syncthreads();
+            const double *xb = Xs[b & 1];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                const int p = 32 * kq + i;
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) part[c] = fma(l[i], xb[p * NCV + c], part[c]);
+            }
+            j = fwd ? j + 1 : j - 1;
+        }
+
+        /* acc = B_s - sum of the four partial products, then the diagonal
+         * step; P and Acc were last read before the previous barriers */
+#pragma unroll
+        for (int c = 0; c < NCV; ++c) P[kq][r * NCV + c] = part[c];
+        __syncthreads();
+        if (kq == 0) {
+#pragma unroll
+            for (int c = 0; c < NCV; ++c) {
+                const int e = r * NCV + c;
+                Acc[e] = bval[c] - (((P[0][e] + P[1][e]) + P[2][e]) + P[3][e]);
+            }
+        }
+        __syncthreads();
+        double xp[NCV];
+#pragma unroll
+        for (int c = 0; c < NCV; ++c) xp[c] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const int p = 32 * kq + i;
+#pragma unroll
+            for (int c = 0; c < NCV; ++c) xp[c] = fma(dreg[i], Acc[p * NCV + c], xp[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NCV; ++c) P[kq][r * NCV + c] = xp[c];
+        __syncthreads();
+        if (kq == 0) {
+#pragma unroll
+            for (int c = 0; c < NCV; ++c) {
+                const int e = r * NCV + c;
+                const double x = ((P[0][e] + P[1][e]) + P[2][e]) + P[3][e];
+                __hip_atomic_store(&Bs[c + (size_t)r * nrp], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        /* publication by completion, as in strip_solve_kernel */
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&a.flags[s], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int NCV>
+static void narrow_launch(const NarrowArgs &a, hipStream_t s)
+{
+    static int per_cu = -1, ncu = 0;
+    if (per_cu < 0) {
+        int dev = 0, pf = 0, pb = 0;
+        CHX_HIP(hipGetDevice(&dev));
+        CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pf, strip_narrow_kernel<NCV, true>, 512, 0));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pb, strip_narrow_kernel<NCV, false>, 512, 0));
+        per_cu = pf < pb ? pf : pb;
+        if (per_cu < 1) per_cu = 1;
+    }
+    int grid = ncu * per_cu;
+    if (grid > a.S - a.s0) grid = a.S - a.s0;
+    zero_ints(reinterpret_cast<int *>(a.ticket), 1, s);
+    if (a.trans)
+        hipLaunchKernelGGL((strip_narrow_kernel<NCV, false>), dim3(grid), dim3(512), 0, s, a);
+    else
+        hipLaunchKernelGGL((strip_narrow_kernel<NCV, true>), dim3(grid), dim3(512), 0, s, a);
+}
+
+/* D128 of strip s (rows 128 s .. 128 s + 127, inside diagonal tile
+ * tm = s / (ldt/128)) from the 64 x 64 inverses Da, Db of its two halves
+ * and the factor block L_ba between them:
+ *     [ L_aa   0   ]^-1   [ Da               0  ]
+ *     [ L_ba  L_bb ]    = [ -Db (L_ba Da)    Db ]
+ * The inverses' upper triangles are taken as zero.  One workgroup per
+ * strip, each thread a 4 x 4 block of the two 64 x 64 products. */
+__global__ __launch_bounds__(256) void dinv128_kernel(const double *A, const double *D, long long dts,
+                                                      long long doff, int ldd, int NT, int ldt, double *D128)
+{
+    __shared__ double X[64 * 65], Y[64 * 65];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int q = ldt / NSR, tm = s / q, rs = (s % q) * NSR;
+    const int qa = ldt / 64, sa = 2 * s, sb = 2 * s + 1;
+    const size_t ts = (size_t)ldt * ldt;
+    const double *Da = D + (size_t)(sa / qa) * dts + (size_t)(sa % qa) * doff;
+    const double *Db = D + (size_t)(sb / qa) * dts + (size_t)(sb % qa) * doff;
+    const double *Lba = A + ((size_t)tm * NT + tm) * ts + (rs + 64) + (size_t)rs * ldt;
+    double *O = D128 + (size_t)s * NSR * NSR;
+    for (int e = tid; e < 4096; e += 256) {
+        const int i = e & 63, k = e >> 6;
+        X[i + k * 65] = Lba[i + (size_t)k * ldt];
+        Y[i + k * 65] = i >= k ? Da[i + (size_t)k * ldd] : 0.0;
+    }
+    __syncthreads();
+    const int i0 = (tid & 15) * 4, j0 = (tid >> 4) * 4;
+    double g[4][4] = {};
+    for (int k = 0; k < 64; ++k) {
+        double x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            x[u] = X[(i0 + u) + k * 65];
+            y[u] = Y[k + (j0 + u) * 65];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) g[u][v] = fma(x[u], y[v], g[u][v]);
+    }
+    /* top half of the result: [Da 0] */
+    for (int e = tid; e < 64 * NSR; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        O[i + (size_t)j * NSR] = j < 64 ? Y[i + j * 65] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) X[(i0 + u) + (j0 + v) * 65] = g[u][v];
+    for (int e = tid; e < 4096; e += 256) {
+        const int i = e & 63, k = e >> 6;
+        Y[i + k * 65] = i >= k ? Db[i + (size_t)k * ldd] : 0.0;
+    }
+    __syncthreads();
+    double c[4][4] = {};
+    for (int k = 0; k < 64; ++k) {
+        double y[4], x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            y[u] = Y[(i0 + u) + k * 65];
+            x[u] = X[k + (j0 + u) * 65];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) c[u][v] = fma(y[u], x[v], c[u][v]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) O[(64 + i0 + u) + (size_t)(j0 + v) * NSR] = -c[u][v];
+    for (int e = tid; e < 4096; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        O[(64 + i) + (size_t)(64 + j) * NSR] = Y[i + j * 65];
+    }
+}
+
+bool narrow_eligible(int ldt, int ncols) { return ldt % NSR == 0 && ncols >= 1 && ncols <= 8; }
+
+void dinv128(const double *A, const double *D, long long dts, long long doff, int ldd, int NT, int ldt,
+             double *D128, hipStream_t s)
+{
+    const int S = NT * ldt / NSR;
+    if (S <= 0) return;
+    hipLaunchKernelGGL(dinv128_kernel, dim3(S), dim3(256), 0, s, A, D, dts, doff, ldd, NT, ldt, D128);
+}
+
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
                  int ldt, int nrp, int trans, int *flags, int epoch, unsigned *ticket, const int *info, hipStream_t s,
-                 int s0)
+                 int s0, const double *D128, int ncols)
 {
+    if (D128 && narrow_eligible(ldt, ncols)) {
+        NarrowArgs na;
+        na.A = A; na.D128 = D128; na.BT = BT; na.flags = flags; na.ticket = ticket;
+        na.NT = NT; na.ldt = ldt; na.nrp = nrp;
+        na.S = NT * ldt / NSR;
+        na.s0 = trans ? 0 : s0 / 2;     /* s0 counts 64-row strips; ldt / 64 is even here */
+        if (na.s0 >= na.S) return 0;
+        na.trans = trans;
+        na.epoch = epoch;
+        na.info = info;
+        if (ncols == 1) narrow_launch<1>(na, s);
+        else if (ncols == 2) narrow_launch<2>(na, s);
+        else if (ncols <= 4) narrow_launch<4>(na, s);
+        else narrow_launch<8>(na, s);
+        return 1;
+    }
     StripArgs a;
     a.A = A; a.D = D; a.dts = dts; a.doff = doff; a.ldd = ldd;
     a.BT = BT; a.flags = flags; a.ticket = ticket;

@@ -209,8 +209,11 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         const bool adaptive = prec == CHAMAMD_PREC_ADAPTIVE;
         const int npanel = adaptive ? 8 : 4;
         const size_t mapbytes = adaptive ? (((size_t)NT * NT + 255) & ~(size_t)255) : 0;
+        /* with few right-hand sides, the 128 x 128 strip inverses of the
+         * first solve (beside the factorisation) at the end */
+        const size_t n128 = k::narrow_eligible(ldt, X->n) ? (size_t)NT * ldt * 128 : 0;
         char *w = (char *)workspace2(abytes + npanel * pbytes + mapbytes +
-                                     (3 * vb + nrp + pv + rows + (size_t)NT * NT) * sizeof(double));
+                                     (3 * vb + nrp + pv + rows + (size_t)NT * NT + n128) * sizeof(double));
         double *Aw = (double *)w;
         unsigned char *dmap = (unsigned char *)(w + abytes + npanel * pbytes);
         double *BT = (double *)(w + abytes + npanel * pbytes + mapbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb,
@@ -246,6 +249,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         p.nvalid_last = N - (NT - 1) * A->mb;
         p.d_info = st->d_info;
         p.lp_mode = prec;
+        p.d128 = n128 ? tss + (size_t)NT * NT : nullptr;
         for (int i = 0; i < 4; ++i) p.P[i] = w + abytes + i * pbytes;
         if (adaptive) {
             for (int i = 0; i < 4; ++i) p.P2[i] = w + abytes + (4 + i) * pbytes;
@@ -268,7 +272,7 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         if (info != 0 || *st->h_info == LP_RANGE) {
             it_count = -3;
         } else {
-            SolvePlan sp = solve_plan(Aw, NT, ldt, nrp, st->d_info);
+            SolvePlan sp = solve_plan(Aw, NT, ldt, nrp, st->d_info, X->n);
             if (!overlap) solve_bt(sp, Aw, NT, ldt, XT, nrp, st->d_info);
             if (scaled) k::scale_bt(XT, nrp, rows, sc, s);
             for (int it = 0; it <= ITERMAX; ++it) {

@@ -149,6 +149,7 @@ struct Job {
     int stage = 0, iter = 0;
     int nrp = 0, rows = 0;
     double *BT = nullptr, *XT = nullptr, *RT = nullptr, *part = nullptr, *nw = nullptr, *rowsum = nullptr;
+    double *d128 = nullptr;           /* 128 x 128 strip inverses of a narrow mixed solve    */
 };
 
 struct Slot {
@@ -537,13 +538,20 @@ void launch_large_solve(Server *s, Slot &sl, Job *job)
     Factor *f = q->f;
     const int nrp = rup16(std::max(q->r.nrhs, 1)), rows = f->NT * f->ldt;
     const size_t vb = (size_t)nrp * rows;
-    char *dv = slot_dev(sl, vb * sizeof(double));
+    /* few right-hand sides: the narrow strip solve, with the 128 x 128
+     * strip inverses formed here from the factor's 64 x 64 ones */
+    const bool narrow = k::narrow_eligible(f->ldt, q->r.nrhs);
+    Layout d;
+    const size_t o_bt = d.add(vb * 8), o_128 = d.add(narrow ? (size_t)rows * 128 * 8 : 0);
+    char *dv = slot_dev(sl, d.used);
     char *h = slot_host(sl, vb * sizeof(double));
     pack_bt(f, q->r, nrp, (double *)h);
-    double *BT = (double *)dv;
+    double *BT = (double *)(dv + o_bt), *d128 = narrow ? (double *)(dv + o_128) : nullptr;
     CHX_HIP(hipMemcpyAsync(BT, h, vb * sizeof(double), hipMemcpyHostToDevice, st));
     k::zero_ints(sl.d_info, 1, st);
-    solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, BT, nrp, sl.d_info);
+    if (narrow)
+        k::dinv128(fptr(s, f), dptr(s, f), (long long)(f->ldt / 64) * 4096, 4096, 64, f->NT, f->ldt, d128, st);
+    solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, BT, nrp, sl.d_info, d128, q->r.nrhs);
     CHX_HIP(hipMemcpyAsync(h, BT, vb * sizeof(double), hipMemcpyDeviceToHost, st));
     job->nrp = nrp;
     job->rows = rows;
@@ -563,12 +571,17 @@ void mixed_step(Server *s, Slot &sl, Job *job, bool first)
         CHX_HIP(hipMemcpyAsync(job->XT, job->BT, vb * sizeof(double), hipMemcpyDeviceToDevice, st));
         k::zero_ints(sl.d_info, 1, st);
         k::scale_bt(job->XT, job->nrp, job->rows, sc, st);
-        solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, job->XT, job->nrp, sl.d_info);
+        if (job->d128)
+            k::dinv128(fptr(s, f), dptr(s, f), (long long)(f->ldt / 64) * 4096, 4096, 64, f->NT, f->ldt, job->d128,
+                       st);
+        solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, job->XT, job->nrp, sl.d_info, job->d128,
+                          q->r.nrhs);
         k::scale_bt(job->XT, job->nrp, job->rows, sc, st);
     } else {
         /* correction: X += A_lp^-1 R */
         k::scale_bt(job->RT, job->nrp, job->rows, sc, st);
-        solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, job->RT, job->nrp, sl.d_info);
+        solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, job->RT, job->nrp, sl.d_info, job->d128,
+                          q->r.nrhs);
         k::scale_bt(job->RT, job->nrp, job->rows, sc, st);
         k::axpy(job->XT, job->RT, vb, st);
     }
@@ -590,9 +603,12 @@ void launch_mixed_solve(Server *s, Slot &sl, Job *job)
     job->rows = f->NT * f->ldt;
     const size_t vb = (size_t)job->nrp * job->rows;
     Layout d;
+    const bool narrow = k::narrow_eligible(f->ldt, q->r.nrhs);
     const size_t o_bt = d.add(vb * 8), o_xt = d.add(vb * 8), o_rt = d.add(vb * 8),
-                 o_part = d.add(k::residual_scratch(f->NT, f->ldt, job->nrp) * 8), o_nw = d.add(2 * job->nrp * 8);
+                 o_part = d.add(k::residual_scratch(f->NT, f->ldt, job->nrp) * 8), o_nw = d.add(2 * job->nrp * 8),
+                 o_128 = d.add(narrow ? (size_t)job->rows * 128 * 8 : 0);
     char *dv = slot_dev(sl, d.used);
+    job->d128 = narrow ? (double *)(dv + o_128) : nullptr;
     job->BT = (double *)(dv + o_bt);
     job->XT = (double *)(dv + o_xt);
     job->RT = (double *)(dv + o_rt);

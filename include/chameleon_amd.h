@@ -261,14 +261,16 @@ int CHAMAMD_Server_Reset_Stats(CHAMAMD_server_t *server);
  * for callers that cannot allocate device memory (code under stream
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * arguments, kernels and results as CHAMELEON_dpotrs_Tile_Async, with the
- * solve's scratch (64 x 64 diagonal-block inverses, transposed right-hand
- * sides, publication flags) in a fixed device array of the library
- * (32 MiB + 64 KiB per device, placed when CHAMELEON_Init loads the code
- * object and held until the process ends) instead of allocated
- * workspaces.  A and B must already be resident on the device (left there
- * by an earlier _Tile_Async call), the padded tile size a multiple of 64,
- * and N_padded * (64 + NRHS_padded) <= 4194304 (e.g. N = 32768 with
- * NRHS <= 64); otherwise CHAMELEON_ERR_ILLEGAL_VALUE or
+ * solve's scratch (64 x 64 diagonal-block inverses, 128 x 128 strip
+ * inverses of a narrow solve, transposed right-hand sides, publication
+ * flags) in a fixed device array of the library (64 MiB + 64 KiB per
+ * device, placed when CHAMELEON_Init loads the code object and held until
+ * the process ends) instead of allocated workspaces.  A and B must already
+ * be resident on the device (left there by an earlier _Tile_Async call),
+ * the padded tile size a multiple of 64, and N_padded * (64 + NRHS_padded,
+ * + 128 with 1 to 8 right-hand sides and tiles a multiple of 128) <=
+ * 8388608 (e.g. N = 32768 with NRHS <= 192, or 1 to 8); otherwise
+ * CHAMELEON_ERR_ILLEGAL_VALUE or
  * CHAMELEON_ERR_OUT_OF_RESOURCES, nothing launched.  Calls that overlap in
  * time on one device share the array and race. */
 int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B,

@@ -344,8 +344,34 @@ def test_lookahead_schedules_agree(ch):
         assert oracle.check_factorization(A0, F, L_)[0]
 
 
+@pytest.mark.parametrize("n,nb,nrhs", [(3000, 128, 1), (2500, 256, 2), (1000, 128, 3), (4100, 384, 4),
+                                       (2048, 256, 6), (3333, 512, 8)])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_narrow_solves_vs_oracle(ch, n, nb, nrhs, uplo):
+    """One to eight right-hand sides with tiles a multiple of 128 take the
+    narrow strip solve (128-row strips, vector FMAs, 128 x 128 strip
+    inverses built from the 64 x 64 ones): dpotrf then dpotrs against the
+    oracle's dposv, and Chameleon's residual check."""
+    say("narrow solves", n, nb, nrhs, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n) == 0
+    assert ch.lib.CHAMELEON_dpotrs(uplo, n, nrhs, _p(A), n, _p(B), n) == 0
+    oracle.use_blas(True)
+    try:
+        Ar, Br, info = reference_solution(uplo, n, nrhs, nb)
+    finally:
+        oracle.use_blas(False)
+    assert info == 0
+    assert oracle.rel_frobenius(B, Br) <= TOL_SOLVE
+    for j in range(nrhs):
+        assert oracle.rel_frobenius(B[:, j:j + 1], Br[:, j:j + 1]) <= TOL_SOLVE, j
+    assert oracle.check_solve(oracle.dplgsy(float(n), UL_, n, 51), B, oracle.dplrnt(n, nrhs, 5673))[0]
+
+
 @pytest.mark.parametrize("n,nb,nrhs", [(500, 512, 7), (1700, 256, 64), (1345, 192, 130), (2000, 320, 1),
-                                       (1536, 128, 48)])
+                                       (1536, 128, 48), (3000, 256, 1), (2600, 512, 4), (3000, 384, 2)])
 @pytest.mark.parametrize("uplo", [L_, U_])
 def test_posv_forward_beside_factorisation(ch, n, nb, nrhs, uplo):
     """dposv solves tile rows forward right after their panels (beside the

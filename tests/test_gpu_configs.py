@@ -137,6 +137,33 @@ def test_fp64_configuration_large(ch, n):
     assert rel_frob_lower(F, Fr) <= 1e-12
 
 
+@pytest.mark.parametrize("nrhs", [1, 8])
+def test_narrow_solve_n32768(ch, nrhs):
+    """dpotrs at N = 32768, NB = 512 with 1 and 8 right-hand sides (the
+    narrow strip solve) on device-resident descriptors: the same columns
+    solved among 16 (the MFMA strip solve) agree to 1e-12, and the scaled
+    residual computed on the device is <= 60."""
+    n, nb = 32768, 512
+    say("config narrow dpotrs", n, nb, nrhs)
+    L = ch.lib
+    ch.set_param(ch.TILE_SIZE, nb)
+    with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB, ch.Desc.alloc(n, 16, nb) as dW:
+        seq = ch.Sequence()
+        ch.check(L.CHAMELEON_dplgsy_Tile_Async(float(n), L_, dA.ptr, 51, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dpotrf_Tile_Async(L_, dA.ptr, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dplrnt_Tile_Async(dW.ptr, 5673, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dpotrs_Tile_Async(L_, dA.ptr, dB.ptr, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dpotrs_Tile_Async(L_, dA.ptr, dW.ptr, seq.ptr, seq.req))
+        assert seq.wait() == 0
+        seq.destroy()
+        X = dB.to_array(n, nrhs)
+        W = dW.to_array(n, 16)
+    assert oracle.rel_frobenius(X, W[:, :nrhs]) <= 1e-12
+    be = device_backward_error(ch, n, nb, X)
+    assert 0.0 < be <= 60.0, be
+
+
 def test_mixed_configuration_n32768(ch):
     """kappa 1e4, NRHS 1: the split-BF16 solve refined in FP64 matches the
     FP64 dposv of the same A and b to the conditioning of the problem, and

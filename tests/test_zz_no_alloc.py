@@ -66,7 +66,7 @@ def solve_both(ch, n, nrhs, nb):
         return dB1.to_array(n, nrhs), dB2.to_array(n, nrhs)
 
 
-@pytest.mark.parametrize("n,nrhs,nb", [(4096, 64, 512), (1000, 7, 128)])
+@pytest.mark.parametrize("n,nrhs,nb", [(4096, 64, 512), (1000, 7, 128), (3000, 1, 256)])
 def test_noalloc_dpotrs_equals_dpotrs(ch, n, nrhs, nb):
     """Two calls in a row (this parametrisation runs the larger first)."""
     say("noalloc dpotrs", n, nrhs, nb)
@@ -87,8 +87,8 @@ def test_noalloc_dpotrs_rejections(ch):
     with ch.Desc.alloc(1024, 1024, nb) as dA, ch.Desc.alloc(1024, 4, nb) as dB:
         rc, st = run(ch, lambda q: L.CHAMAMD_dpotrs_Tile_Async_NoAlloc(L_, dA.ptr, dB.ptr, q.ptr, q.req))
         assert rc == ch.ERR_ILLEGAL_VALUE and st == rc
-    # larger than the array: 16384 * (64 + 208) doubles > 2^22
-    n, nrhs = 16384, 200
+    # larger than the array: 16384 * (64 + 512) doubles > 2^23
+    n, nrhs = 16384, 500
     with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB:
         rc, st = run(ch, lambda q: (L.CHAMELEON_dplgsy_Tile_Async(float(n), L_, dA.ptr, 51, q.ptr, q.req) or
                                     L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673, q.ptr, q.req)))
