@@ -489,9 +489,17 @@ struct NarrowArgs {
 template <int NCV, bool FWD>
 __global__ __launch_bounds__(512, 1) void strip_narrow_kernel(NarrowArgs a)
 {
+    /* backward: one factor block at a time goes through LDS (rows of TLD
+     * doubles: 16-byte reads of 8 consecutive rows hit distinct banks), and
+     * the diagonal step's buffers reuse that space once the blocks are done */
+    constexpr int TLD = NSR + 2;
+    constexpr int PSZ = 5 * NSR * NCV;
+    constexpr int SMEM = FWD ? PSZ : (NSR * TLD > PSZ ? NSR * TLD : PSZ);
     __shared__ double Xs[2][NSR * NCV];   /* published blocks, element (p, c) at p*NCV + c */
-    __shared__ double P[4][NSR * NCV];    /* partial sums of the four k ranges               */
-    __shared__ double Acc[NSR * NCV];     /* input of the diagonal step                       */
+    __shared__ double smem[SMEM];
+    double *const T = smem;               /* backward: op(r, p) at T[r*TLD + p]               */
+    double (*const P)[NSR * NCV] = reinterpret_cast<double (*)[NSR * NCV]>(smem);   /* partial sums */
+    double *const Acc = smem + 4 * NSR * NCV;                                        /* diagonal input */
     __shared__ int s_task;
     if (*a.info != 0) return;     /* uniform over the grid at launch */
     /* wave and k range are uniform over a wavefront: scalar registers, so
@@ -517,20 +525,19 @@ __global__ __launch_bounds__(512, 1) void strip_narrow_kernel(NarrowArgs a)
             return fwd ? a.A + ((size_t)tj * a.NT + tm) * ts + rs + (size_t)rj * ldt
                        : a.A + ((size_t)tm * a.NT + tj) * ts + rj + (size_t)rs * ldt;
         };
-        /* op(r, p) of block j for p in this thread's k range */
+        /* forward: op(r, p) = L(s, j)(r, p) of block j for p in this
+         * thread's k range (lanes on consecutive r: coalesced) */
         auto load_l = [&](int j, double (&l)[32]) {
             const double *b = block_of(j);
-            if (fwd) {
 #pragma unroll
-                for (int i = 0; i < 32; ++i) l[i] = b[r + (size_t)(32 * kq + i) * ldt];
-            } else {
+            for (int i = 0; i < 32; ++i) l[i] = b[r + (size_t)(32 * kq + i) * ldt];
+        };
+        /* backward: op(r, p) = L(j, s)(p, r) is contiguous in p; wave w
+         * reads rows r = 16 w .. 16 w + 15, one 1 KB row per instruction */
+        auto load_g = [&](int j, v2d (&g)[16]) {
+            const double *b = block_of(j);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const v4d v = *(const v4d *)(b + 32 * kq + 4 * i + (size_t)r * ldt);
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) l[4 * i + e] = v[e];
-                }
-            }
+            for (int i = 0; i < 16; ++i) g[i] = *(const v2d *)(b + 2 * lane + (size_t)(16 * wave + i) * ldt);
         };
         double bval[NCV];
         if (kq == 0) {
@@ -608,21 +615,50 @@ __global__ __launch_bounds__(512, 1) void strip_narrow_kernel(NarrowArgs a)
          * flight per workgroup (128 KB) is enough to keep HBM busy */
         const int nblk = fwd ? s - a.s0 : a.S - 1 - s;
         int j = fwd ? a.s0 : a.S - 1;
-        for (int b = 0; b < nblk; ++b) {
-            double l[32];
-            load_l(j, l);
-            /* Xs[b&1] was last read in iteration b-2, before the barrier of
-             * iteration b-1 */
-            if (wave == 0) fetch(j, b & 1);
-            __syncthreads();
-            const double *xb = Xs[b & 1];
+        if constexpr (FWD) {
+            for (int b = 0; b < nblk; ++b) {
+                double l[32];
+                load_l(j, l);
+                /* Xs[b&1] was last read in iteration b-2, before the barrier
+                 * of iteration b-1 */
+                if (wave == 0) fetch(j, b & 1);
+                __syncthreads();
+                const double *xb = Xs[b & 1];
 #pragma unroll
-            for (int i = 0; i < 32; ++i) {
-                const int p = 32 * kq + i;
+                for (int i = 0; i < 32; ++i) {
+                    const int p = 32 * kq + i;
 #pragma unroll
-                for (int c = 0; c < NCV; ++c) part[c] = fma(l[i], xb[p * NCV + c], part[c]);
+                    for (int c = 0; c < NCV; ++c) part[c] = fma(l[i], xb[p * NCV + c], part[c]);
+                }
+                ++j;
             }
-            j = fwd ? j + 1 : j - 1;
+        } else {
+            /* block j in registers (coalesced) -> T -> row reads; the next
+             * block's loads are issued before this block's products */
+            v2d g[16];
+            if (nblk > 0) load_g(j, g);
+            for (int b = 0; b < nblk; ++b) {
+                if (wave == 0) fetch(j, b & 1);
+                __syncthreads();                 /* T no longer read; Xs[b&1] ready */
+#pragma unroll
+                for (int i = 0; i < 16; ++i) *(v2d *)&T[(16 * wave + i) * TLD + 2 * lane] = g[i];
+                __syncthreads();                 /* T holds block j */
+                if (b + 1 < nblk) load_g(j - 1, g);
+                const double *xb = Xs[b & 1];
+                const double *tr = T + r * TLD + 32 * kq;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const v2d t = *(const v2d *)(tr + 2 * i);
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int p = 32 * kq + 2 * i + e;
+#pragma unroll
+                        for (int c = 0; c < NCV; ++c) part[c] = fma(t[e], xb[p * NCV + c], part[c]);
+                    }
+                }
+                --j;
+            }
+            __syncthreads();                     /* T no longer read: P and Acc reuse it */
         }
 
         /* acc = B_s - sum of the four partial products, then the diagonal
