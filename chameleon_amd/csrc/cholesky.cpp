@@ -232,18 +232,24 @@ SolvePlan solve_plan(double *A, int NT, int ldt, int nrp, int *d_info, int ncols
     p.ncols = ncols;
     const bool narrow = k::narrow_eligible(ldt, ncols);
     const size_t ntile_inv = p.R == 64 ? 0 : 2;
+    /* the narrow solve's 128 x 128 strip inverses, or the MFMA strip
+     * solve's predecessor products (both directions) */
     const size_t n128 = narrow ? (size_t)NT * ldt * 128 : 0;
-    const size_t need = (ntile_inv * NT * ts + (size_t)nrp * NT * ldt + (size_t)nrp * ldt + (size_t)NT * dper + n128) *
-                        sizeof(double);
+    const size_t npe = (!narrow && p.R == 64) ? 2 * (size_t)NT * ldt * 64 : 0;
+    const size_t need = (ntile_inv * NT * ts + (size_t)nrp * NT * ldt + (size_t)nrp * ldt + (size_t)NT * dper + n128 +
+                         npe) * sizeof(double);
     double *ws = (double *)workspace(need);
     p.BT = ws;
     p.W = p.BT + (size_t)nrp * NT * ldt;
     p.dinv = p.W + (size_t)nrp * ldt;
     p.d128 = narrow ? p.dinv + (size_t)NT * dper : nullptr;
-    p.TT = ntile_inv ? p.dinv + (size_t)NT * dper + n128 : nullptr;
+    p.ef = npe ? p.dinv + (size_t)NT * dper + n128 : nullptr;
+    p.eb = npe ? p.ef + (size_t)NT * ldt * 64 : nullptr;
+    p.TT = ntile_inv ? p.dinv + (size_t)NT * dper + n128 + npe : nullptr;
     p.TN = ntile_inv ? p.TT + (size_t)NT * ts : nullptr;
     k::dinv_all(A, NT, (size_t)(NT + 1) * ts, ldt, p.dinv, sm);
     if (narrow) k::dinv128(A, p.dinv, (long long)dper, 64LL * 64, 64, NT, ldt, p.d128, sm);
+    if (npe) k::strip_products(A, p.dinv, (long long)dper, 64LL * 64, 64, NT, ldt, p.ef, p.eb, sm);
     if (p.R == 64) return p;
     k::set_identity(p.TT, NT, ldt, sm);
     k::trsm_right_lower_trans_batched(p.TT, NT, (long long)ts, ldt, A, (long long)(NT + 1) * ts, p.dinv,
@@ -270,7 +276,7 @@ static void solve_pair(const SolvePlan &p, double *A, int NT, int ldt, double *B
     /* 64 x 64 inverse of block b of diagonal tile t at dinv + (t*nob + b)*4096, ld 64 */
     for (int dir = 0; dir < 2; ++dir)
         k::solve_strips(A, p.dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
-                        panel_counters() + 2, d_info, sm, 0, p.d128, p.ncols);
+                        panel_counters() + 2, d_info, sm, 0, p.d128, p.ncols, dir ? p.eb : p.ef);
 }
 
 /* In-place solve of transposed right-hand sides BT (nrp x NT*ldt) with the
@@ -281,7 +287,7 @@ void solve_bt(const SolvePlan &p, double *A, int NT, int ldt, double *BT, int nr
 }
 
 void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int ldt, double *BT, int nrp,
-                       const int *d_info, const double *d128, int ncols)
+                       const int *d_info, const double *d128, int ncols, const double *ef, const double *eb)
 {
     const int S = NT * ldt / 64, chunks = (nrp + 63) / 64;
     int epoch = 0;
@@ -289,7 +295,7 @@ void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int
     const size_t dper = (size_t)(ldt / 64) * 64 * 64;
     for (int dir = 0; dir < 2; ++dir)
         k::solve_strips(A, dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
-                        lane_counters(L) + 2, d_info, L->sm, 0, d128, ncols);
+                        lane_counters(L) + 2, d_info, L->sm, 0, d128, ncols, dir ? eb : ef);
 }
 
 /* B <- A^-1 B with the Lower factor in A: forward then backward solve on
@@ -490,11 +496,14 @@ void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate)
     const bool narrow = p.d128 && k::narrow_eligible(ldt, nrhs);
     if (narrow) k::dinv128(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, NT, ldt, p.d128, sm);
     const double *d128 = narrow ? p.d128 : nullptr;
+    /* otherwise the predecessor products of the MFMA strip solve */
+    double *ef = (!narrow && p.epred) ? p.epred : nullptr, *eb = ef ? ef + (size_t)NT * ldt * 64 : nullptr;
+    if (ef) k::strip_products(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, NT, ldt, ef, eb, sm);
     timed(CHAMAMD_KERNEL_SOLVE, (rows_left * rows_left + n * n) * nrhs, sm, [&] {
         k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 0, flags, epoch,
-                        lane_counters(L) + 2, p.d_info, sm, s0, d128, nrhs);
+                        lane_counters(L) + 2, p.d_info, sm, s0, d128, nrhs, ef);
         k::solve_strips(f.A, f.dinv, (long long)f.dper, 64LL * 64, 64, f.BT, NT, ldt, nrp, 1, flags,
-                        epoch + chunks, lane_counters(L) + 2, p.d_info, sm, 0, d128, nrhs);
+                        epoch + chunks, lane_counters(L) + 2, p.d_info, sm, 0, d128, nrhs, eb);
     });
 }
 
@@ -521,7 +530,8 @@ int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequenc
      * potrf_run puts them), the transposed right-hand sides, and with few
      * right-hand sides the 128 x 128 strip inverses of the solves */
     const size_t n128 = k::narrow_eligible(ldt, B->n) ? (size_t)NT * ldt * 128 : 0;
-    double *ws = (double *)workspace(((size_t)NT * dper + (size_t)nrp * NT * ldt + n128) * sizeof(double));
+    const size_t npe = n128 ? 0 : 2 * (size_t)NT * ldt * 64;      /* predecessor products */
+    double *ws = (double *)workspace(((size_t)NT * dper + (size_t)nrp * NT * ldt + n128 + npe) * sizeof(double));
     double *BT = ws + (size_t)NT * dper;
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, sm);
 
@@ -533,6 +543,7 @@ int posv_async(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, RUNTIME_sequenc
     p.nvalid_last = A->m - (NT - 1) * A->mb;
     p.d_info = st->d_info;
     p.d128 = n128 ? BT + (size_t)nrp * NT * ldt : nullptr;
+    p.epred = npe ? BT + (size_t)nrp * NT * ldt : nullptr;
     potrf_solve_run(p, BT, nrp, B->n, 60e12);
     st->pending_info = true;
     mirror_mark_written(A);

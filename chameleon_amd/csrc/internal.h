@@ -235,13 +235,17 @@ struct PotrfPlan {
     /* potrf_solve_run with at most 8 right-hand sides: room for the 128 x 128
      * strip inverses (NT*ldt*128 doubles), or null for the MFMA strip solve */
     double *d128 = nullptr;
+    /* potrf_solve_run, MFMA strip solve: room for the predecessor products
+     * (2 x NT*ldt*64 doubles), or null to solve without them */
+    double *epred = nullptr;
 };
 void potrf_run(const PotrfPlan &p);
 /* both strip solves of BT (nrp x NT*ldt, transposed right-hand sides) with
  * the Lower factor A and its 64 x 64 diagonal-block inverses, on lane L;
  * ldt a multiple of 64 */
 void solve_strips_lane(Lane *L, const double *A, const double *dinv, int NT, int ldt, double *BT, int nrp,
-                       const int *d_info, const double *d128 = nullptr, int ncols = 0);
+                       const int *d_info, const double *d128 = nullptr, int ncols = 0, const double *ef = nullptr,
+                       const double *eb = nullptr);
 /* potrf_run with the forward solve of BT (transposed right-hand sides,
  * nrp x NT*ldt, nrhs of them in use) issued beside the factorisation, then
  * the forward strips it left and the backward solve: BT <- A^-1 BT.  The
@@ -253,6 +257,7 @@ void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate);
 struct SolvePlan {
     double *TT, *TN, *BT, *W, *dinv;   /* diagonal-tile inverses, transposed RHS, one block, 64x64 inverses */
     double *d128 = nullptr;            /* 128 x 128 strip inverses (narrow solves), or null */
+    double *ef = nullptr, *eb = nullptr;   /* predecessor products (MFMA strip solve), or null */
     int ncols = 0;                     /* right-hand-side columns in use */
     int R = 0;                         /* 64: strip solve; 0: tile-step solve */
 };
@@ -381,9 +386,15 @@ void tile_gemm_nt(const double *A, int amt, const double *B, int bmt, double *C,
  * use with ldt a multiple of 128, the narrow kernel solves the first ncols
  * columns only (128-row strips, vector FMAs); otherwise the MFMA strip
  * kernel solves all nrp columns. */
+/* E (optional, MFMA kernel only): the predecessor products of this
+ * direction from strip_products (Ef forward, Eb backward). */
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
                  int ldt, int nrp, int trans, int *flags, int epoch, unsigned *ticket, const int *info, hipStream_t s,
-                 int s0 = 0, const double *D128 = nullptr, int ncols = 0);
+                 int s0 = 0, const double *D128 = nullptr, int ncols = 0, const double *E = nullptr);
+/* predecessor products of the 64-row strips (NT*ldt*64 doubles each at Ef
+ * and Eb): Ef_s = D_s L(s, s-1), Eb_s = (L(s+1, s) D_s)^T */
+void strip_products(const double *A, const double *D, long long dts, long long doff, int ldd, int NT, int ldt,
+                    double *Ef, double *Eb, hipStream_t s);
 bool narrow_eligible(int ldt, int ncols);
 /* 128 x 128 inverses of the diagonal blocks of the 128-row strips (NT*ldt*128
  * doubles at D128) from the 64 x 64 inverses D (addressed as in solve_strips) */

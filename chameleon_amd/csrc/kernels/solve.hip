@@ -206,6 +206,10 @@ struct StripArgs {
                            * already subtracted from the right-hand sides)           */
     int trans, epoch;
     const int *info;
+    /* null, or the predecessor products E_s (strip_products) of this
+     * direction: strip s's 64 x 64 E_s at E + s*4096, element (r, p) at
+     * r + p*64 */
+    const double *E;
 };
 
 /* operand of lane (li, lk) for k-chunk p0 of a 16-row group: element
@@ -228,7 +232,7 @@ __device__ __forceinline__ void load_op4(const double *M, int ld, bool vec, int 
 /* polls of one flag wait before it gives up: about a second at ~0.3 us per
  * poll, far beyond any chain (a full direction at N = 32768 is ~3 ms) */
 #define STRIP_SPIN_LIMIT (1 << 22)
-template <int NC>
+template <int NC, bool PE>
 __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
 {
     constexpr int NCOL = 16 * NC, NPC = SR / 16;
@@ -344,8 +348,14 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
         /* blocks j in chain order; the last one is the chain predecessor.
          * Block j is multiplied before the wait for block j+1, so a strip
          * at the front of the chain has its partial sum ready when its
-         * predecessor publishes. */
-        const int nblk = fwd ? s - a.s0 : a.S - 1 - s;
+         * predecessor publishes.  With E the predecessor is left out here:
+         * X_s = D_s acc' - E_s X_pred, where acc' lacks the predecessor's
+         * block and E_s = D_s L(s, s-1) (forward) or D_s^T L(s+1, s)^T
+         * (backward), so the diagonal product runs before the predecessor
+         * publishes and a chain link holds one product instead of two. */
+        const int nall = fwd ? s - a.s0 : a.S - 1 - s;
+        const bool use_e = PE && nall > 0;
+        const int nblk = use_e ? nall - 1 : nall;
         if (nblk > 0) {
             double lcur[NPC][4], lnext[NPC][4];
             v2d xv[XV];
@@ -391,6 +401,14 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
             }
         }
 
+        /* the predecessor product's operand, requested before the
+         * diagonal step (its latency hides behind it) */
+        double ev[NPC][4];
+        if constexpr (PE) {
+            const double *Eg = a.E + (size_t)s * SR * SR;
+#pragma unroll
+            for (int pc = 0; pc < NPC; ++pc) load_op4(Eg, SR, false, wr, pc * 16, li, lk, ev[pc]);
+        }
         /* diagonal block: X_s = D_s acc (forward) or D_s^T acc (backward) */
 #pragma unroll
         for (int c = 0; c < NC; ++c)
@@ -408,6 +426,23 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv, Xs[p * NCOL + c * 16 + li], x[c]);
             }
+        }
+        if (PE && use_e) {
+            /* X_s -= E_s X_pred once the predecessor is published */
+            const int jp = fwd ? s - 1 : s + 1;
+            v2d xv[XV];
+            wait_for(jp);
+            stage_load(jp, xv);
+            __syncthreads();                     /* every wave is done with Xs */
+            stage_store(xv);
+            __syncthreads();
+#pragma unroll
+            for (int pc = 0; pc < NPC; ++pc)
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                        x[c] = mfma_f64(-ev[pc][qq], Xb[(pc * 16 + 4 * lk + qq) * NCOL + c * 16 + li], x[c]);
         }
         {
             /* publication by completion: X_s is written with device-scope
@@ -430,7 +465,7 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
     }
 }
 
-template <int NC>
+template <int NC, bool PE>
 static void strip_launch(const StripArgs &a, hipStream_t s)
 {
     static int per_cu = -1, ncu = 0;
@@ -438,13 +473,13 @@ static void strip_launch(const StripArgs &a, hipStream_t s)
         int dev = 0;
         CHX_HIP(hipGetDevice(&dev));
         CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, strip_solve_kernel<NC>, 256, 0));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, strip_solve_kernel<NC, PE>, 256, 0));
         if (per_cu < 1) per_cu = 1;
     }
     int grid = ncu * per_cu;
     if (grid > (a.S - a.s0) * a.nchunk) grid = (a.S - a.s0) * a.nchunk;
     zero_ints(reinterpret_cast<int *>(a.ticket), 1, s);
-    hipLaunchKernelGGL(strip_solve_kernel<NC>, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((strip_solve_kernel<NC, PE>), dim3(grid), dim3(256), 0, s, a);
 }
 
 /* ------------------------------------------------------------------ */
@@ -810,9 +845,97 @@ void dinv128(const double *A, const double *D, long long dts, long long doff, in
     hipLaunchKernelGGL(dinv128_kernel, dim3(S), dim3(256), 0, s, A, D, dts, doff, ldd, NT, ldt, D128);
 }
 
+/* Predecessor products of the 64-row strips (strip s = rows 64 s ..):
+ *   forward  Ef_s = D_s L(s, s-1)          for s >= 1
+ *   backward Eb_s = (L(s+1, s) D_s)^T      for s <= S-2
+ * with D_s the 64 x 64 inverse of the diagonal block (upper triangle taken
+ * as zero), each 64 x 64 at E + s*4096, element (r, p) at r + p*64.  One
+ * workgroup per strip, each thread a 4 x 4 block of the products. */
+__global__ __launch_bounds__(256) void strip_products_kernel(const double *A, const double *D, long long dts,
+                                                             long long doff, int ldd, int NT, int ldt, double *Ef,
+                                                             double *Eb)
+{
+    __shared__ double Dl[64 * 65], Lb[64 * 65];
+    const int s = blockIdx.x, tid = threadIdx.x, S = NT * ldt / SR;
+    const int q = ldt / SR;
+    const size_t ts = (size_t)ldt * ldt;
+    const double *Dg = D + (size_t)(s / q) * dts + (size_t)(s % q) * doff;
+    auto block = [&](int row_strip, int col_strip) -> const double * {
+        const int tr = row_strip / q, rr = (row_strip % q) * SR, tc = col_strip / q, rc = (col_strip % q) * SR;
+        return A + ((size_t)tc * NT + tr) * ts + rr + (size_t)rc * ldt;   /* element (i, k) at i + k*ldt */
+    };
+    for (int e = tid; e < 4096; e += 256) {
+        const int i = e & 63, k = e >> 6;
+        Dl[i + k * 65] = i >= k ? Dg[i + (size_t)k * ldd] : 0.0;
+    }
+    const int i0 = (tid & 15) * 4, j0 = (tid >> 4) * 4;
+    if (s >= 1) {
+        const double *L = block(s, s - 1);
+        for (int e = tid; e < 4096; e += 256) {
+            const int i = e & 63, k = e >> 6;
+            Lb[i + k * 65] = L[i + (size_t)k * ldt];
+        }
+        __syncthreads();
+        double g[4][4] = {};
+        for (int k = 0; k < 64; ++k) {
+            double d[4], l[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                d[u] = Dl[(i0 + u) + k * 65];
+                l[u] = Lb[k + (j0 + u) * 65];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) g[u][v] = fma(d[u], l[v], g[u][v]);
+        }
+        double *O = Ef + (size_t)s * 4096;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) O[(i0 + u) + (j0 + v) * 64] = g[u][v];
+    }
+    if (s <= S - 2) {
+        const double *L = block(s + 1, s);
+        __syncthreads();                  /* Lb no longer read */
+        for (int e = tid; e < 4096; e += 256) {
+            const int i = e & 63, k = e >> 6;
+            Lb[i + k * 65] = L[i + (size_t)k * ldt];
+        }
+        __syncthreads();
+        /* G = L(s+1, s) D_s; Eb = G^T: Eb(r, p) = G(p, r) */
+        double g[4][4] = {};
+        for (int k = 0; k < 64; ++k) {
+            double l[4], d[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                l[u] = Lb[(i0 + u) + k * 65];
+                d[u] = Dl[k + (j0 + u) * 65];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) g[u][v] = fma(l[u], d[v], g[u][v]);
+        }
+        double *O = Eb + (size_t)s * 4096;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) O[(j0 + v) + (i0 + u) * 64] = g[u][v];
+    }
+}
+
+void strip_products(const double *A, const double *D, long long dts, long long doff, int ldd, int NT, int ldt,
+                    double *Ef, double *Eb, hipStream_t s)
+{
+    const int S = NT * ldt / SR;
+    if (S <= 0) return;
+    hipLaunchKernelGGL(strip_products_kernel, dim3(S), dim3(256), 0, s, A, D, dts, doff, ldd, NT, ldt, Ef, Eb);
+}
+
 int solve_strips(const double *A, const double *D, long long dts, long long doff, int ldd, double *BT, int NT,
                  int ldt, int nrp, int trans, int *flags, int epoch, unsigned *ticket, const int *info, hipStream_t s,
-                 int s0, const double *D128, int ncols)
+                 int s0, const double *D128, int ncols, const double *E)
 {
     if (D128 && narrow_eligible(ldt, ncols)) {
         NarrowArgs na;
@@ -839,6 +962,7 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
     if (a.s0 >= a.S) return 0;
     a.trans = trans;
     a.info = info;
+    a.E = E;
     /* groups of up to 64 columns per launch; a full group is split into
      * two chunks of 32 columns that run as separate chains side by side
      * (shorter work per chain link).  Each chunk fetches the factor rows
@@ -857,11 +981,20 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
             nc = snc;
             a.nchunk = 4 / snc;
         }
-        switch (nc) {
-        case 1: strip_launch<1>(a, s); break;
-        case 2: strip_launch<2>(a, s); break;
-        case 3: strip_launch<3>(a, s); break;
-        default: strip_launch<4>(a, s); break;
+        /* the predecessor products for chunks of 16 and 32 columns (wider
+         * chunks would not fit their registers) */
+        if (nc <= 2 && E) {
+            if (nc == 1) strip_launch<1, true>(a, s);
+            else strip_launch<2, true>(a, s);
+        } else {
+            a.E = nullptr;
+            switch (nc) {
+            case 1: strip_launch<1, false>(a, s); break;
+            case 2: strip_launch<2, false>(a, s); break;
+            case 3: strip_launch<3, false>(a, s); break;
+            default: strip_launch<4, false>(a, s); break;
+            }
+            a.E = E;
         }
     }
     return used;

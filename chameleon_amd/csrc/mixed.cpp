@@ -209,9 +209,10 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         const bool adaptive = prec == CHAMAMD_PREC_ADAPTIVE;
         const int npanel = adaptive ? 8 : 4;
         const size_t mapbytes = adaptive ? (((size_t)NT * NT + 255) & ~(size_t)255) : 0;
-        /* with few right-hand sides, the 128 x 128 strip inverses of the
-         * first solve (beside the factorisation) at the end */
-        const size_t n128 = k::narrow_eligible(ldt, X->n) ? (size_t)NT * ldt * 128 : 0;
+        /* at the end, for the first solve (beside the factorisation): with
+         * few right-hand sides the 128 x 128 strip inverses, else the
+         * predecessor products of the MFMA strip solve */
+        const size_t n128 = k::narrow_eligible(ldt, X->n) ? (size_t)NT * ldt * 128 : 2 * (size_t)NT * ldt * 64;
         char *w = (char *)workspace2(abytes + npanel * pbytes + mapbytes +
                                      (3 * vb + nrp + pv + rows + (size_t)NT * NT + n128) * sizeof(double));
         double *Aw = (double *)w;
@@ -249,7 +250,10 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         p.nvalid_last = N - (NT - 1) * A->mb;
         p.d_info = st->d_info;
         p.lp_mode = prec;
-        p.d128 = n128 ? tss + (size_t)NT * NT : nullptr;
+        if (k::narrow_eligible(ldt, X->n))
+            p.d128 = tss + (size_t)NT * NT;
+        else
+            p.epred = tss + (size_t)NT * NT;     /* predecessor products of the first solve */
         for (int i = 0; i < 4; ++i) p.P[i] = w + abytes + i * pbytes;
         if (adaptive) {
             for (int i = 0; i < 4; ++i) p.P2[i] = w + abytes + (4 + i) * pbytes;

@@ -3,10 +3,10 @@
  *
  * CHAMAMD_dpotrs_Tile_Async_NoAlloc runs the same kernels, in the same
  * order, as CHAMELEON_dpotrs_Tile_Async with tiles a multiple of 64 (the
- * 64 x 64 diagonal-block inverses, with at most 8 right-hand sides and
- * tiles a multiple of 128 the 128 x 128 strip inverses, the transposition
- * of B, one strip-solve launch per direction, the transposition back), but
- * takes every buffer
+ * 64 x 64 diagonal-block inverses, then with at most 8 right-hand sides and
+ * tiles a multiple of 128 the 128 x 128 strip inverses, else the strip
+ * solve's predecessor products, the transposition of B, one strip-solve
+ * launch per direction, the transposition back), but takes every buffer
  * beyond A and B from `scratch`, a device array of fixed size that the
  * code object carries, instead of the library's growable workspaces.  It
  * is for callers that cannot call hipMalloc: code under stream capture,
@@ -21,13 +21,14 @@
  *                                 each launch)
  *   [NOALLOC_FLAG_BYTES, ...)     doubles: the 64 x 64 inverses
  *                                 (Npad * 64), the 128 x 128 strip
- *                                 inverses of a narrow solve (Npad * 128,
- *                                 else nothing), then the transposed
+ *                                 inverses of a narrow solve or the two
+ *                                 directions' predecessor products
+ *                                 (Npad * 128), then the transposed
  *                                 right-hand sides (nrp * Npad)
  * with Npad = NT * ldt (padded rows) and nrp = B's padded column count.
  * Calls are rejected (CHAMELEON_ERR_OUT_OF_RESOURCES) when that exceeds
- * NOALLOC_MAX_DOUBLES (64 MiB of doubles), e.g. N = 32768 is served with
- * up to 192 right-hand sides, or with 1 to 8 on the narrow path.
+ * NOALLOC_MAX_DOUBLES (128 MiB of doubles), e.g. N = 32768 is served with
+ * up to 320 right-hand sides.
  *
  * Every call on a device shares that device's array, which keeps the last
  * call's contents: calls that overlap in time on one device race (issue
@@ -41,7 +42,7 @@
 using namespace chx;
 
 #define NOALLOC_FLAG_BYTES (64u << 10)
-#define NOALLOC_MAX_DOUBLES ((size_t)1 << 23)      /* 64 MiB of doubles */
+#define NOALLOC_MAX_DOUBLES ((size_t)1 << 24)      /* 128 MiB of doubles */
 #define NOALLOC_MAX_DEVICES 64
 
 static __device__ __align__(256) unsigned char scratch[NOALLOC_FLAG_BYTES + NOALLOC_MAX_DOUBLES * sizeof(double)];
@@ -137,11 +138,12 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
     const size_t npad = (size_t)NT * ldt;
     const int nrp = MB->nt * MB->wdt;
     const bool narrow = k::narrow_eligible(ldt, B->n);
-    const size_t ndinv = npad * 64, n128 = narrow ? npad * 128 : 0, nbt = (size_t)nrp * npad;
+    /* the narrow solve's strip inverses or the MFMA solve's predecessor
+     * products: Npad * 128 doubles either way */
+    const size_t ndinv = npad * 64, naux = npad * 128, nbt = (size_t)nrp * npad;
     const int S = (int)(npad / 64);
-    if (ndinv + n128 + nbt > NOALLOC_MAX_DOUBLES || (size_t)(4 * S + 1) * sizeof(int) > NOALLOC_FLAG_BYTES) {
-        error(fn, "problem larger than the fixed scratch array (N_padded * (64 + NRHS_padded, + 128 with at most "
-                  "8 right-hand sides) <= 8388608)");
+    if (ndinv + naux + nbt > NOALLOC_MAX_DOUBLES || (size_t)(4 * S + 1) * sizeof(int) > NOALLOC_FLAG_BYTES) {
+        error(fn, "problem larger than the fixed scratch array (N_padded * (192 + NRHS_padded) <= 16777216)");
         return request_fail(sequence, request, CHAMELEON_ERR_OUT_OF_RESOURCES);
     }
     hipStream_t s = c->s_main;
@@ -168,8 +170,10 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
      * every flag (at most 4 * S <= 4096 of them) */
     unsigned *ticket = (unsigned *)(d->ptr + NOALLOC_FLAG_BYTES) - 1;
     double *dinv = (double *)(d->ptr + NOALLOC_FLAG_BYTES);
-    double *d128 = narrow ? dinv + ndinv : nullptr;
-    double *BT = dinv + ndinv + n128;
+    double *aux = dinv + ndinv;
+    double *d128 = narrow ? aux : nullptr;
+    double *ef = narrow ? nullptr : aux, *eb = narrow ? nullptr : aux + npad * 64;
+    double *BT = dinv + ndinv + naux;
     SeqState *st = seq_state(sequence);
 
     if (MB->state != MIRROR_NORMAL) {
@@ -193,11 +197,14 @@ extern "C" int CHAMAMD_dpotrs_Tile_Async_NoAlloc(cham_uplo_t uplo, CHAM_desc_t *
     const double *Ad = (const double *)MA->d;
     const size_t ts = (size_t)ldt * ldt, dper = (size_t)(ldt / 64) * 64 * 64;
     k::dinv_all(Ad, NT, (size_t)(NT + 1) * ts, ldt, dinv, s);
-    if (narrow) k::dinv128(Ad, dinv, (long long)dper, 64LL * 64, 64, NT, ldt, d128, s);
+    if (narrow)
+        k::dinv128(Ad, dinv, (long long)dper, 64LL * 64, 64, NT, ldt, d128, s);
+    else
+        k::strip_products(Ad, dinv, (long long)dper, 64LL * 64, 64, NT, ldt, ef, eb, s);
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
     for (int dir = 0; dir < 2; ++dir)
         k::solve_strips(Ad, dinv, (long long)dper, 64LL * 64, 64, BT, NT, ldt, nrp, dir, flags, epoch + dir * chunks,
-                        ticket, st->d_info, s, 0, d128, B->n);
+                        ticket, st->d_info, s, 0, d128, B->n, dir ? eb : ef);
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 0, s);
     st->pending_info = true;
     mirror_mark_written(B);

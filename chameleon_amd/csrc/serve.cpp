@@ -150,6 +150,7 @@ struct Job {
     int nrp = 0, rows = 0;
     double *BT = nullptr, *XT = nullptr, *RT = nullptr, *part = nullptr, *nw = nullptr, *rowsum = nullptr;
     double *d128 = nullptr;           /* 128 x 128 strip inverses of a narrow mixed solve    */
+    double *ef = nullptr, *eb = nullptr;   /* predecessor products of a wide mixed solve     */
 };
 
 struct Slot {
@@ -539,19 +540,26 @@ void launch_large_solve(Server *s, Slot &sl, Job *job)
     const int nrp = rup16(std::max(q->r.nrhs, 1)), rows = f->NT * f->ldt;
     const size_t vb = (size_t)nrp * rows;
     /* few right-hand sides: the narrow strip solve, with the 128 x 128
-     * strip inverses formed here from the factor's 64 x 64 ones */
+     * strip inverses formed here from the factor's 64 x 64 ones; else the
+     * MFMA strip solve with its predecessor products, formed here too */
     const bool narrow = k::narrow_eligible(f->ldt, q->r.nrhs);
     Layout d;
-    const size_t o_bt = d.add(vb * 8), o_128 = d.add(narrow ? (size_t)rows * 128 * 8 : 0);
+    const size_t o_bt = d.add(vb * 8), o_aux = d.add((size_t)rows * 128 * 8);
     char *dv = slot_dev(sl, d.used);
     char *h = slot_host(sl, vb * sizeof(double));
     pack_bt(f, q->r, nrp, (double *)h);
-    double *BT = (double *)(dv + o_bt), *d128 = narrow ? (double *)(dv + o_128) : nullptr;
+    double *BT = (double *)(dv + o_bt), *aux = (double *)(dv + o_aux);
     CHX_HIP(hipMemcpyAsync(BT, h, vb * sizeof(double), hipMemcpyHostToDevice, st));
     k::zero_ints(sl.d_info, 1, st);
-    if (narrow)
-        k::dinv128(fptr(s, f), dptr(s, f), (long long)(f->ldt / 64) * 4096, 4096, 64, f->NT, f->ldt, d128, st);
-    solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, BT, nrp, sl.d_info, d128, q->r.nrhs);
+    const long long dper = (long long)(f->ldt / 64) * 4096;
+    if (narrow) {
+        k::dinv128(fptr(s, f), dptr(s, f), dper, 4096, 64, f->NT, f->ldt, aux, st);
+        solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, BT, nrp, sl.d_info, aux, q->r.nrhs);
+    } else {
+        double *eb = aux + (size_t)rows * 64;
+        k::strip_products(fptr(s, f), dptr(s, f), dper, 4096, 64, f->NT, f->ldt, aux, eb, st);
+        solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, BT, nrp, sl.d_info, nullptr, 0, aux, eb);
+    }
     CHX_HIP(hipMemcpyAsync(h, BT, vb * sizeof(double), hipMemcpyDeviceToHost, st));
     job->nrp = nrp;
     job->rows = rows;
@@ -571,17 +579,19 @@ void mixed_step(Server *s, Slot &sl, Job *job, bool first)
         CHX_HIP(hipMemcpyAsync(job->XT, job->BT, vb * sizeof(double), hipMemcpyDeviceToDevice, st));
         k::zero_ints(sl.d_info, 1, st);
         k::scale_bt(job->XT, job->nrp, job->rows, sc, st);
+        const long long dper = (long long)(f->ldt / 64) * 4096;
         if (job->d128)
-            k::dinv128(fptr(s, f), dptr(s, f), (long long)(f->ldt / 64) * 4096, 4096, 64, f->NT, f->ldt, job->d128,
-                       st);
+            k::dinv128(fptr(s, f), dptr(s, f), dper, 4096, 64, f->NT, f->ldt, job->d128, st);
+        else
+            k::strip_products(fptr(s, f), dptr(s, f), dper, 4096, 64, f->NT, f->ldt, job->ef, job->eb, st);
         solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, job->XT, job->nrp, sl.d_info, job->d128,
-                          q->r.nrhs);
+                          q->r.nrhs, job->ef, job->eb);
         k::scale_bt(job->XT, job->nrp, job->rows, sc, st);
     } else {
         /* correction: X += A_lp^-1 R */
         k::scale_bt(job->RT, job->nrp, job->rows, sc, st);
         solve_strips_lane(&sl.lane, fptr(s, f), dptr(s, f), f->NT, f->ldt, job->RT, job->nrp, sl.d_info, job->d128,
-                          q->r.nrhs);
+                          q->r.nrhs, job->ef, job->eb);
         k::scale_bt(job->RT, job->nrp, job->rows, sc, st);
         k::axpy(job->XT, job->RT, vb, st);
     }
@@ -606,9 +616,13 @@ void launch_mixed_solve(Server *s, Slot &sl, Job *job)
     const bool narrow = k::narrow_eligible(f->ldt, q->r.nrhs);
     const size_t o_bt = d.add(vb * 8), o_xt = d.add(vb * 8), o_rt = d.add(vb * 8),
                  o_part = d.add(k::residual_scratch(f->NT, f->ldt, job->nrp) * 8), o_nw = d.add(2 * job->nrp * 8),
-                 o_128 = d.add(narrow ? (size_t)job->rows * 128 * 8 : 0);
+                 o_aux = d.add((size_t)job->rows * 128 * 8);
     char *dv = slot_dev(sl, d.used);
-    job->d128 = narrow ? (double *)(dv + o_128) : nullptr;
+    /* the narrow solve's strip inverses, or the MFMA solve's predecessor
+     * products (formed in the first step, read by every correction) */
+    job->d128 = narrow ? (double *)(dv + o_aux) : nullptr;
+    job->ef = narrow ? nullptr : (double *)(dv + o_aux);
+    job->eb = narrow ? nullptr : job->ef + (size_t)job->rows * 64;
     job->BT = (double *)(dv + o_bt);
     job->XT = (double *)(dv + o_xt);
     job->RT = (double *)(dv + o_rt);

@@ -262,14 +262,14 @@ int CHAMAMD_Server_Reset_Stats(CHAMAMD_server_t *server);
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * arguments, kernels and results as CHAMELEON_dpotrs_Tile_Async, with the
  * solve's scratch (64 x 64 diagonal-block inverses, 128 x 128 strip
- * inverses of a narrow solve, transposed right-hand sides, publication
- * flags) in a fixed device array of the library (64 MiB + 64 KiB per
- * device, placed when CHAMELEON_Init loads the code object and held until
- * the process ends) instead of allocated workspaces.  A and B must already
- * be resident on the device (left there by an earlier _Tile_Async call),
- * the padded tile size a multiple of 64, and N_padded * (64 + NRHS_padded,
- * + 128 with 1 to 8 right-hand sides and tiles a multiple of 128) <=
- * 8388608 (e.g. N = 32768 with NRHS <= 192, or 1 to 8); otherwise
+ * inverses of a narrow solve or the strip solve's predecessor products,
+ * transposed right-hand sides, publication flags) in a fixed device array
+ * of the library (128 MiB + 64 KiB per device, placed when CHAMELEON_Init
+ * loads the code object and held until the process ends) instead of
+ * allocated workspaces.  A and B must already be resident on the device
+ * (left there by an earlier _Tile_Async call), the padded tile size a
+ * multiple of 64, and N_padded * (192 + NRHS_padded) <= 16777216 (e.g.
+ * N = 32768 with NRHS <= 320); otherwise
  * CHAMELEON_ERR_ILLEGAL_VALUE or
  * CHAMELEON_ERR_OUT_OF_RESOURCES, nothing launched.  Calls that overlap in
  * time on one device share the array and race. */

@@ -87,8 +87,8 @@ def test_noalloc_dpotrs_rejections(ch):
     with ch.Desc.alloc(1024, 1024, nb) as dA, ch.Desc.alloc(1024, 4, nb) as dB:
         rc, st = run(ch, lambda q: L.CHAMAMD_dpotrs_Tile_Async_NoAlloc(L_, dA.ptr, dB.ptr, q.ptr, q.req))
         assert rc == ch.ERR_ILLEGAL_VALUE and st == rc
-    # larger than the array: 16384 * (64 + 512) doubles > 2^23
-    n, nrhs = 16384, 500
+    # larger than the array: 32768 * (192 + 512) doubles > 2^24
+    n, nrhs = 32768, 500
     with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB:
         rc, st = run(ch, lambda q: (L.CHAMELEON_dplgsy_Tile_Async(float(n), L_, dA.ptr, 51, q.ptr, q.req) or
                                     L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673, q.ptr, q.req)))
