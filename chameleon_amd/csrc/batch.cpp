@@ -620,7 +620,8 @@ int CHAMAMD_dposv_mixed_batch(CHAMAMD_batch_t *b, int prec, int *iter, int *info
         CHX_HIP(hipMalloc(&b->d_state, count * sizeof(int)));
         CHX_HIP(hipMalloc(&b->d_iters, count * sizeof(int)));
     }
-    CHX_HIP(hipMemcpyAsync(b->dW, b->dA, b->a_elems * sizeof(double), hipMemcpyDeviceToDevice, sm));
+    /* the working copy: only the part the factorisation touches */
+    k::batch_copy_lower(b->dA, b->dW, b->d_aoff, b->d_np, count, sm);
     if (b->b_elems) CHX_HIP(hipMemcpyAsync(b->dX, b->dB, b->b_elems * sizeof(double), hipMemcpyDeviceToDevice, sm));
     issue_potrf(b, g_bucket_waves, b->dW, true);
     k::zero_ints(b->d_queue + 1, 1, sm);

@@ -331,6 +331,9 @@ void batch_solve(const double *A, const long long *aoff, const int *np, const in
                  int *queue, int grid, hipStream_t s);
 void zero_ints(int *p, int n, hipStream_t s);
 /* refinement pieces of the batched mixed-precision solve (batch.hip) */
+/* dst = src on the lower 64-column blocks of every system (the part the
+ * batched factorisation reads and writes) */
+void batch_copy_lower(const double *src, double *dst, const long long *aoff, const int *np, int count, hipStream_t s);
 void batch_residual(const double *A, const long long *aoff, const int *np, const int *n, const double *B,
                     const double *X, double *R, const long long *boff, const int *nrhs, double *anrm, int *state,
                     int *iters, int *unconv, double *berr, int count, int *queue, hipStream_t s);
@@ -360,6 +363,9 @@ void residual(const double *A, int NT, int ldt, const double *XT, const double *
               hipStream_t s);
 size_t residual_scratch(int NT, int ldt, int nrp);   /* doubles of `part` */
 void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStream_t s);
+/* column maxima of X into out[0..ncols) and of R into out[nrp..nrp+ncols)
+ * (max-combined into out: clear it first) */
+void colmax2(const double *X, const double *R, int nrp, int ncols, int nrows, double *out, hipStream_t s);
 void axpy(double *Y, const double *X, size_t n, hipStream_t s);
 /* lower tiles of S copied to W and the symmetric inf-norm's row sums into
  * rowsum[N]; returns -1 (nothing launched) unless ldt is a multiple of 128

@@ -212,12 +212,14 @@ void batch_solve(const double *A, const long long *aoff, const int *np, const in
  *   backward error max_c ||R_c|| / (||A|| ||X_c|| n eps) goes to berr[j].
  *   ||A||_inf (largest row sum of |A|) is formed in the same pass when
  *   anrm[j] < 0.
- * One workgroup of 256 threads per system (queue); thread t owns rows
- * t, t + 256, ...: row i takes A(i, 0..i) (consecutive threads read
- * consecutive rows: coalesced) and A(i+1.., i) down its own column, n
- * elements for every row.  Right-hand sides go in chunks of 8 columns;
- * the chunk's X rows are staged in LDS 1024 rows at a time (row-major, 8
- * wide).  Every sum runs in a fixed order: deterministic. */
+ * One workgroup of 4 wavefronts per system (queue).  A wavefront's unit is
+ * a 16-row x 16-column block of A X, formed with FP64 MFMAs over k in
+ * chunks of 16 (lane (li, lk) supplies k = p0 + 4 lk + q in product q):
+ * chunks left of the block's diagonal read A as stored (16 consecutive
+ * rows per load), chunks right of it read the stored lower triangle
+ * transposed (each lane 4 consecutive k of one column: a 32-byte load),
+ * the diagonal chunk element by element.  Each sum runs in a fixed order
+ * and the maxima are order-free, so the result is deterministic. */
 struct BatchResArgs {
     const double *A;
     const long long *aoff;
@@ -233,14 +235,24 @@ struct BatchResArgs {
     int *queue;
 };
 
+#define RES_CB 256       /* right-hand-side columns per pass of a system */
+
+__device__ __forceinline__ void lds_max_bits(double *p, double v)
+{
+    /* non-negative doubles order like their bits; a NaN (made positive)
+     * compares above every number, so the maximum propagates it */
+    unsigned long long u = (unsigned long long)__double_as_longlong(fabs(v));
+    if (v != v) u = 0x7ff8000000000000ull;
+    atomicMax(reinterpret_cast<unsigned long long *>(p), u);
+}
+
 __global__ __launch_bounds__(256) void batch_residual_kernel(BatchResArgs a)
 {
-    constexpr int JB = 1024, CW = 8, RG = 1024;
-    __shared__ double Xs[JB * CW];
+    __shared__ double colr[RES_CB], colx[RES_CB];
     __shared__ double red[256];
-    __shared__ double colr[CW], colx[CW];
     __shared__ int s_job;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
     const double eps = 0x1p-53;
     for (;;) {
         __syncthreads();
@@ -254,72 +266,73 @@ __global__ __launch_bounds__(256) void batch_residual_kernel(BatchResArgs a)
         const double *B = a.B + a.boff[j], *X = a.X + a.boff[j];
         double *R = a.R + a.boff[j];
         const bool want_norm = a.anrm[j] < 0.0;
+        const int ngr = (n + 15) / 16;             /* 16-row groups; 16 * ngr <= np */
         double an = want_norm ? 0.0 : a.anrm[j];
-        double worst = 0.0;
+        double rowmax = 0.0, worst = 0.0;
         bool ok = true;
-        for (int c0 = 0; c0 < nr; c0 += CW) {
-            const int cw = min(CW, nr - c0);
-            const bool norm_pass = want_norm && c0 == 0;
-            double rmax[CW], xmax[CW], rowmax = 0.0;
+        for (int cb = 0; cb < nr; cb += RES_CB) {
+            const int ncb = min(RES_CB, nr - cb), nfr = (ncb + 15) / 16;
+            const bool norm_pass = want_norm && cb == 0;
+            for (int e = tid; e < RES_CB; e += 256) colr[e] = colx[e] = 0.0;
+            __syncthreads();
+            for (int u = wave; u < ngr * nfr; u += 4) {
+                const int g = u % ngr, f = u / ngr;
+                const int r0 = 16 * g, i = r0 + li;        /* A row this lane supplies */
+                const int cl = 16 * f + li;                /* column of the pass, B operand */
+                const bool cv = cl < ncb;
+                const double *Xc = X + (size_t)(cb + (cv ? cl : 0)) * np;
+                v4d acc = zero4d();
+                double rs = 0.0;
+                for (int p0 = 0; p0 < 16 * ngr; p0 += 16) {
+                    double av[4];
+                    if (p0 + 16 <= r0) {
 #pragma unroll
-            for (int c = 0; c < CW; ++c) rmax[c] = xmax[c] = 0.0;
-            for (int rg = 0; rg < n; rg += RG) {
-                double acc[RG / 256][CW], rowabs[RG / 256];
+                        for (int q = 0; q < 4; ++q) av[q] = A[i + (size_t)(p0 + 4 * lk + q) * np];
+                    } else if (p0 >= r0 + 16) {
+                        const v4d t4 = *(const v4d *)(A + (p0 + 4 * lk) + (size_t)i * np);
 #pragma unroll
-                for (int q = 0; q < RG / 256; ++q) {
-                    rowabs[q] = 0.0;
+                        for (int q = 0; q < 4; ++q) av[q] = t4[q];
+                    } else {
 #pragma unroll
-                    for (int c = 0; c < CW; ++c) acc[q][c] = 0.0;
-                }
-                for (int jb = 0; jb < n; jb += JB) {
-                    const int je = min(n, jb + JB), len = je - jb;
-                    __syncthreads();
-                    for (int e = tid; e < len * CW; e += 256) {
-                        const int c = e / len, jj = e - c * len;
-                        Xs[jj * CW + c] = c < cw ? X[(jb + jj) + (size_t)(c0 + c) * np] : 0.0;
-                    }
-                    __syncthreads();
-#pragma unroll
-                    for (int q = 0; q < RG / 256; ++q) {
-                        const int i = rg + tid + 256 * q;
-                        if (i >= n) break;
-                        /* row part: A(i, jb .. min(i, je-1)) */
-                        const int rend = min(i + 1, je);
-                        for (int jj = jb; jj < rend; ++jj) {
-                            const double v = A[i + (size_t)jj * np];
-                            if (norm_pass) rowabs[q] += fabs(v);
-                            const double *xs = &Xs[(jj - jb) * CW];
-#pragma unroll
-                            for (int c = 0; c < CW; ++c) acc[q][c] += v * xs[c];
-                        }
-                        /* column part: A(max(i+1, jb) .. je-1, i) */
-                        const double *col = A + (size_t)i * np;
-                        for (int jj = max(i + 1, jb); jj < je; ++jj) {
-                            const double v = col[jj];
-                            if (norm_pass) rowabs[q] += fabs(v);
-                            const double *xs = &Xs[(jj - jb) * CW];
-#pragma unroll
-                            for (int c = 0; c < CW; ++c) acc[q][c] += v * xs[c];
+                        for (int q = 0; q < 4; ++q) {
+                            const int k = p0 + 4 * lk + q;
+                            av[q] = i >= k ? A[i + (size_t)k * np] : A[k + (size_t)i * np];
                         }
                     }
+                    v4d xb = zero4d();
+                    if (cv) xb = *(const v4d *)(Xc + p0 + 4 * lk);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int k = p0 + 4 * lk + q;
+                        if (k >= n) av[q] = 0.0;
+                        if (norm_pass && f == 0) rs += fabs(av[q]);
+                        acc = mfma_f64(av[q], xb[q], acc);
+                    }
                 }
+                if (norm_pass && f == 0) {
+                    /* row i's |A| sum: the four k quarters meet */
+                    rs += __shfl_xor(rs, 16);
+                    rs += __shfl_xor(rs, 32);
+                    if (lk == 0 && i < n) rowmax = nan_max(rowmax, rs);
+                }
+                /* slot s: row r0 + lk + 4 s, column cl */
+                double rm = 0.0, xm = 0.0;
 #pragma unroll
-                for (int q = 0; q < RG / 256; ++q) {
-                    const int i = rg + tid + 256 * q;
-                    if (i >= n) break;
-                    rowmax = nan_max(rowmax, rowabs[q]);
-#pragma unroll
-                    for (int c = 0; c < CW; ++c) {
-                        if (c >= cw) break;
-                        const size_t o = i + (size_t)(c0 + c) * np;
-                        const double r = B[o] - acc[q][c];
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    const int row = r0 + lk + 4 * s2;
+                    if (cv && row < n) {
+                        const size_t o = row + (size_t)(cb + cl) * np;
+                        const double r = B[o] - acc[s2];
                         R[o] = r;
-                        rmax[c] = nan_max(rmax[c], fabs(r));
-                        xmax[c] = nan_max(xmax[c], fabs(X[o]));
+                        rm = nan_max(rm, fabs(r));
+                        xm = nan_max(xm, fabs(X[o]));
                     }
+                }
+                if (cv) {
+                    lds_max_bits(&colr[cl], rm);
+                    lds_max_bits(&colx[cl], xm);
                 }
             }
-            /* workgroup maxima: ||A||_inf (first chunk), then per column */
             if (norm_pass) {
                 __syncthreads();
                 red[tid] = rowmax;
@@ -331,30 +344,13 @@ __global__ __launch_bounds__(256) void batch_residual_kernel(BatchResArgs a)
                 an = red[0];
                 if (tid == 0) a.anrm[j] = an;
             }
-            for (int c = 0; c < cw; ++c) {
-                __syncthreads();
-                red[tid] = rmax[c];
-                __syncthreads();
-                for (int w = 128; w > 0; w >>= 1) {
-                    if (tid < w) red[tid] = nan_max(red[tid], red[tid + w]);
-                    __syncthreads();
-                }
-                if (tid == 0) colr[c] = red[0];
-                __syncthreads();
-                red[tid] = xmax[c];
-                __syncthreads();
-                for (int w = 128; w > 0; w >>= 1) {
-                    if (tid < w) red[tid] = nan_max(red[tid], red[tid + w]);
-                    __syncthreads();
-                }
-                if (tid == 0) colx[c] = red[0];
-            }
             __syncthreads();
-            for (int c = 0; c < cw; ++c) {
+            for (int c = 0; c < ncb; ++c) {
                 if (!(colr[c] <= colx[c] * an * eps * sqrt((double)n))) ok = false;
                 const double d = an * colx[c] * n * eps;
                 worst = nan_max(worst, d > 0.0 ? colr[c] / d : (colr[c] > 0.0 ? INFINITY : 0.0));
             }
+            __syncthreads();
         }
         if (tid == 0) {
             a.berr[j] = worst;
@@ -366,6 +362,36 @@ __global__ __launch_bounds__(256) void batch_residual_kernel(BatchResArgs a)
             }
         }
     }
+}
+
+/* dst = src on the part of every system the factorisation reads: in each
+ * 64-column block, rows from the block's first column down (the lower
+ * triangle and the diagonal 64 x 64 blocks whole).  Workgroup per system,
+ * consecutive threads on consecutive rows. */
+__global__ __launch_bounds__(256) void batch_copy_lower_kernel(const double *src, double *dst, const long long *aoff,
+                                                               const int *np, int count)
+{
+    for (int j = blockIdx.x; j < count; j += gridDim.x) {
+        const int n = np[j];
+        const double *S = src + aoff[j];
+        double *D = dst + aoff[j];
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int h = n - c0, w = min(64, h);
+            const int tot = h * w;
+#pragma unroll 4
+            for (int e = threadIdx.x; e < tot; e += 256) {
+                const int c = c0 + e / h, r = c0 + e % h;
+                D[r + (size_t)c * n] = S[r + (size_t)c * n];
+            }
+        }
+    }
+}
+
+void batch_copy_lower(const double *src, double *dst, const long long *aoff, const int *np, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(batch_copy_lower_kernel, dim3(std::min(count, 8192)), dim3(256), 0, s, src, dst, aoff, np,
+                       count);
 }
 
 void batch_residual(const double *A, const long long *aoff, const int *np, const int *n, const double *B,

@@ -852,6 +852,25 @@ void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStre
     hipLaunchKernelGGL(colmax_kernel, dim3(64, ncols), dim3(256), 0, s, V, nrp, nrows, out);
 }
 
+/* the column maxima of X (out[j]) and of R (out[nrp + j]) in one launch */
+__global__ void colmax2_kernel(const double *X, const double *R, int nrp, int nrows, double *out)
+{
+    const int j = blockIdx.y;
+    const double *V = blockIdx.z ? R : X;
+    double mx = 0.0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nrows; p += gridDim.x * blockDim.x)
+        mx = nan_max(mx, fabs(V[j + (size_t)p * nrp]));
+    for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(&out[blockIdx.z * nrp + j]),
+                  (unsigned long long)__double_as_longlong(mx));
+}
+
+void colmax2(const double *X, const double *R, int nrp, int ncols, int nrows, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(colmax2_kernel, dim3(64, ncols, 2), dim3(256), 0, s, X, R, nrp, nrows, out);
+}
+
 __global__ void axpy_kernel(double *Y, const double *X, size_t n)
 {
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)

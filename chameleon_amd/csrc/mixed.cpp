@@ -63,16 +63,19 @@ double norm_inf_sym(CHAM_desc_t *A)
     return r;
 }
 
-/* column max-norms of a transposed block (nrp x rows) */
-std::vector<double> colmax_host(const double *V, int nrp, int nvalid, int rows, double *dwork)
+/* column maxima of X and R (transposed, nrp x rows) on the host, one
+ * launch and one round trip; dwork holds 2 nrp doubles */
+void colmax_pair_host(const double *XT, const double *RT, int nrp, int rows, double *dwork, std::vector<double> &xn,
+                      std::vector<double> &rn)
 {
-    Context *c = ctx();
-    CHX_HIP(hipMemsetAsync(dwork, 0, nrp * sizeof(double), c->s_main));
-    k::colmax(V, nrp, nvalid, rows, dwork, c->s_main);
-    std::vector<double> h(nvalid);
-    CHX_HIP(hipMemcpyAsync(h.data(), dwork, nvalid * sizeof(double), hipMemcpyDeviceToHost, c->s_main));
-    CHX_HIP(hipStreamSynchronize(c->s_main));
-    return h;
+    hipStream_t s = ctx()->s_main;
+    k::zero_ints((int *)dwork, 4 * nrp, s);
+    k::colmax2(XT, RT, nrp, nrp, rows, dwork, s);
+    std::vector<double> h(2 * (size_t)nrp);
+    CHX_HIP(hipMemcpyAsync(h.data(), dwork, 2 * (size_t)nrp * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHX_HIP(hipStreamSynchronize(s));
+    xn.assign(h.begin(), h.begin() + nrp);
+    rn.assign(h.begin() + nrp, h.end());
 }
 
 /* Per-tile precision of the trailing products, Chameleon's rule for
@@ -159,13 +162,13 @@ int backward_error(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *X, CHAM_desc_t
     const double anrm = norm_inf_sym(A);
     const size_t vb = (size_t)nrp * rows;
     const size_t pv = k::residual_scratch(NT, ldt, nrp);
-    double *w = (double *)workspace2((3 * vb + nrp + pv) * sizeof(double));
-    double *XT = w, *BT = w + vb, *RT = w + 2 * vb, *nw = w + 3 * vb, *part = nw + nrp;
+    double *w = (double *)workspace2((3 * vb + 2 * (size_t)nrp + pv) * sizeof(double));
+    double *XT = w, *BT = w + vb, *RT = w + 2 * vb, *nw = w + 3 * vb, *part = nw + 2 * (size_t)nrp;
     k::rhs_transpose((double *)MX->d, XT, NT, ldt, MX->wdt, MX->nt, 1, s);
     k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);
     k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, part, s);
-    std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
-    std::vector<double> rn = colmax_host(RT, nrp, nrp, rows, nw);
+    std::vector<double> xn, rn;
+    colmax_pair_host(XT, RT, nrp, rows, nw, xn, rn);
     const double eps = std::ldexp(1.0, -53);
     double worst = 0.0;
     for (int j = 0; j < X->n; ++j) {
@@ -214,11 +217,11 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
          * predecessor products of the MFMA strip solve */
         const size_t n128 = k::narrow_eligible(ldt, X->n) ? (size_t)NT * ldt * 128 : 2 * (size_t)NT * ldt * 64;
         char *w = (char *)workspace2(abytes + npanel * pbytes + mapbytes +
-                                     (3 * vb + nrp + pv + rows + (size_t)NT * NT + n128) * sizeof(double));
+                                     (3 * vb + 2 * (size_t)nrp + pv + rows + (size_t)NT * NT + n128) * sizeof(double));
         double *Aw = (double *)w;
         unsigned char *dmap = (unsigned char *)(w + abytes + npanel * pbytes);
         double *BT = (double *)(w + abytes + npanel * pbytes + mapbytes), *XT = BT + vb, *RT = XT + vb, *nw = RT + vb,
-               *part = nw + nrp;
+               *part = nw + 2 * (size_t)nrp;
         double *sc = part + pv;       /* diagonal scaling of the working copy */
         double *tss = sc + rows;      /* tile norms of the per-tile precision choice */
         /* the factorisation reads and writes the lower tiles only: copy
@@ -281,8 +284,8 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
             if (scaled) k::scale_bt(XT, nrp, rows, sc, s);
             for (int it = 0; it <= ITERMAX; ++it) {
                 k::residual((const double *)MA->d, NT, ldt, XT, BT, RT, nrp, part, s);
-                std::vector<double> xn = colmax_host(XT, nrp, nrp, rows, nw);
-                std::vector<double> rn = colmax_host(RT, nrp, nrp, rows, nw);
+                std::vector<double> xn, rn;
+                colmax_pair_host(XT, RT, nrp, rows, nw, xn, rn);
                 bool ok = true;
                 worst = 0.0;
                 for (int j = 0; j < X->n; ++j) {

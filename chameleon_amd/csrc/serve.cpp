@@ -598,8 +598,7 @@ void mixed_step(Server *s, Slot &sl, Job *job, bool first)
     /* FP64 residual on the kept lower tiles of A, column max-norms of X and R */
     k::residual(optr(s, f), f->NT, f->ldt, job->XT, job->BT, job->RT, job->nrp, job->part, st);
     k::zero_ints((int *)job->nw, 4 * job->nrp, st);
-    k::colmax(job->XT, job->nrp, job->nrp, job->rows, job->nw, st);
-    k::colmax(job->RT, job->nrp, job->nrp, job->rows, job->nw + job->nrp, st);
+    k::colmax2(job->XT, job->RT, job->nrp, job->nrp, job->rows, job->nw, st);
     CHX_HIP(hipMemcpyAsync(sl.host + job->host_off, job->nw, 2 * job->nrp * sizeof(double), hipMemcpyDeviceToHost,
                            st));
     job->stage = 1;
