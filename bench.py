@@ -496,6 +496,32 @@ def copy_probe(ch, n, nb, reps=3):
                                      "bytes_written_per_launch": n * n * 8}}), flush=True)
 
 
+def solve_probe(ch, n, nb, nrhs, reps=3):
+    """A few dpotrs calls on a resident factor (N x N, NB, NRHS): a short
+    program for counter passes over the strip solves.  Algorithmic traffic
+    per strip-solve launch: the factor's lower triangle, 8 N (N+1) / 2 B."""
+    L = ch.lib
+    ch.set_param(ch.TILE_SIZE, nb)
+    times = []
+    with ch.Desc.alloc(n, n, nb) as dA, ch.Desc.alloc(n, nrhs, nb) as dB:
+        seq = ch.Sequence()
+        ch.check(L.CHAMELEON_dplgsy_Tile_Async(float(n), 122, dA.ptr, 51, seq.ptr, seq.req))
+        ch.check(L.CHAMELEON_dpotrf_Tile_Async(122, dA.ptr, seq.ptr, seq.req))
+        ch.check(seq.wait())
+        seq.destroy()
+        for _ in range(reps):
+            seq = ch.Sequence()
+            ch.check(L.CHAMELEON_dplrnt_Tile_Async(dB.ptr, 5673, seq.ptr, seq.req))
+            ch.check(seq.wait())
+            t0 = time.perf_counter()
+            ch.check(L.CHAMELEON_dpotrs_Tile_Async(122, dA.ptr, dB.ptr, seq.ptr, seq.req))
+            ch.check(seq.wait())
+            times.append(round((time.perf_counter() - t0) * 1e3, 3))
+            seq.destroy()
+    print(json.dumps({"solve_probe": {"n": n, "nb": nb, "nrhs": nrhs, "launches_per_direction": reps,
+                                      "factor_bytes": 8 * n * (n + 1) // 2, "ms": times}}), flush=True)
+
+
 def host_cpu_info():
     """Cores this process may run on (os.sched_getaffinity), physical cores
     among them (SMT siblings counted once), the OMP_NUM_THREADS cap of the
@@ -612,6 +638,8 @@ def main():
                     help="arrival window of the serving legs (0: skip)")
     ap.add_argument("--no-serve-mixed", dest="serve_mixed", action="store_false",
                     help="skip the serving leg with mixed-precision factors")
+    ap.add_argument("--solve-probe", type=int, default=0, metavar="NRHS",
+                    help="only run 3 dpotrs calls (N, NB 512, NRHS) on a resident factor and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run an N x N device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -624,6 +652,9 @@ def main():
     ch.init()
     if args.copy_probe:
         copy_probe(ch, args.copy_probe, 512)
+        return
+    if args.solve_probe:
+        solve_probe(ch, args.n, 512, args.solve_probe)
         return
     n, nrhs = args.n, args.nrhs
     nb = args.nb
