@@ -14,12 +14,17 @@
 * Batched: the full 20 000-system batch (n log-uniform in [16, 1024],
   nrhs 1..8): every system's scaled residual <= 60, and 50 systems spread
   over the size range against the oracle.
+* Serving: 10 s of Poisson arrivals at 2000 requests/s (n log-uniform in
+  [64, 8192], nrhs in {1, 4, 16}, 30 % reusing an earlier matrix, 200 GB
+  factor budget): every request served with status 0 and every solution's
+  scaled residual <= 60, computed on the device with A regenerated.
 
 Tolerances as in test_gpu_cholesky.py: factor 1e-12, solution 1e-11.
 """
 import ctypes
 import os
 import sys
+import time
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
@@ -252,3 +257,69 @@ def test_batch_configuration_20000(ch):
             oracle.use_blas(False)
     finally:
         ch.lib.CHAMAMD_Batch_Destroy(ctypes.byref(b))
+
+
+def test_serve_configuration_10s(ch):
+    """The serving configuration at its own size (FP64 factors).  Requests
+    arrive open-loop at 2000/s for 10 s; after the drain every request's
+    scaled residual ||B - A X||_inf / (||A||_inf ||X||_inf N eps) is formed
+    on the device (CHAMAMD_Backward_Error, A = dplgsy(bump = n, seed)
+    regenerated once per matrix id) and must be <= 60."""
+    L = ch.lib
+    rng = np.random.default_rng(11)
+    rate, seconds = 2000.0, 10.0
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    reqs, seen = [], []
+    for i in range(len(arrivals)):
+        if seen and rng.random() < 0.3:
+            mid, n = seen[rng.integers(len(seen))]
+        else:
+            n = int(np.floor(np.exp(rng.uniform(np.log(64), np.log(8192)))))
+            mid = i
+            seen.append((mid, n))
+        reqs.append((float(arrivals[i]), mid, n, int(rng.choice([1, 4, 16]))))
+    base = np.asfortranarray(rng.uniform(-0.5, 0.5, (8192, 16)))
+    say("config serve", len(reqs), "requests,", len(seen), "matrices")
+    ch.set_param(ch.TILE_SIZE, 256)
+    srv = ch.Server(200 << 30)
+    out = {}
+    try:
+        t0 = time.perf_counter()
+        nxt = 0
+        while nxt < len(reqs):
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt][0] <= now:
+                _, mid, n, r = reqs[nxt]
+                B = base[:n, :r].copy(order="F")
+                out[srv.submit(mid, n, B, seed=1000 + mid)] = (mid, n, r, B)
+                nxt += 1
+            srv.progress()
+        srv.drain()
+        for t in out:
+            assert srv.test(t) == (True, 0), t
+        st = srv.stats()
+        wall = time.perf_counter() - t0
+    finally:
+        srv.destroy()
+    say("served", st["completed"], "in", round(wall, 1), "s; factored", st["factored"], "reused", st["reused"])
+    assert st["completed"] == len(reqs)
+    by_mid = {}
+    for mid, n, r, X in out.values():
+        by_mid.setdefault((mid, n), []).append((r, X))
+    worst = 0.0
+    for (mid, n), items in by_mid.items():
+        with ch.Desc.alloc(n, n, 256) as dA:
+            seq = ch.Sequence()
+            ch.check(L.CHAMELEON_dplgsy_Tile_Async(float(n), UL_, dA.ptr, 1000 + mid, seq.ptr, seq.req))
+            ch.check(seq.wait())
+            seq.destroy()
+            for r, X in items:
+                assert np.all(np.isfinite(X)), (mid, n)
+                B0 = base[:n, :r].copy(order="F")
+                with ch.Desc.from_array(X, 256) as dX, ch.Desc.from_array(B0, 256) as dB:
+                    be = ctypes.c_double(-1.0)
+                    ch.check(L.CHAMAMD_Backward_Error(L_, dA.ptr, dX.ptr, dB.ptr, ctypes.byref(be)))
+                assert 0.0 <= be.value <= 60.0, (mid, n, r, be.value)
+                worst = max(worst, be.value)
+    say("worst scaled residual", worst)
