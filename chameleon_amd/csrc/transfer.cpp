@@ -10,14 +10,20 @@
  * moved by column panels of one tile column (split to fit a staging
  * chunk), not tile by tile:
  *   upload:   host worker threads copy the panel's columns out of the
- *             user's array into a pinned staging chunk; one contiguous
- *             DMA moves the chunk into device staging; a repack kernel
+ *             user's array into a pinned staging chunk; a repack kernel
+ *             reads the chunk across the bus (coalesced, from every CU at
+ *             once: 54 GB/s, where one DMA engine moved 28 GB/s) and
  *             scatters it into the tiles of the device mirror.  While the
- *             bus and the kernel work on one chunk, the threads fill the
- *             next (three chunks in turn).
- *   download: the reverse: gather kernel, DMA into pinned staging, worker
- *             threads scatter the columns into the user's array, the next
- *             chunks' gathers and DMAs already issued.
+ *             kernel works on one chunk, the threads fill the next (three
+ *             chunks in turn).  CHAMAMD_XFER_ZC=0: a DMA into device
+ *             staging first, then the repack.
+ *   download: a gather kernel packs the tiles' columns into device
+ *             staging, one DMA moves them into the pinned chunk, worker
+ *             threads scatter them into the user's array, the next chunks'
+ *             gathers and DMAs already issued.  The download mostly runs
+ *             beside the factorisation (streamed, below), where kernels
+ *             writing across the bus would hold CUs for the whole transfer;
+ *             the DMA engine moves it without them.
  * Only the rows a triangular operation reads need to move: XFER_LOWER
  * takes each tile column from its diagonal tile down (the lower triangle
  * of tiles, all a Lower Cholesky reads or writes), XFER_UPPER the rest.  Pageable user memory cannot be
@@ -106,9 +112,11 @@ private:
 
 struct Staging {
     char *host[XFER_CHUNKS] = {};
+    char *hdev[XFER_CHUNKS] = {};   /* the same pinned chunk as the device addresses it */
     char *dev[XFER_CHUNKS] = {};
     hipEvent_t ev[XFER_CHUNKS] = {};
     bool used[XFER_CHUNKS] = {};
+    bool direct = true;             /* the upload's repack kernels read the pinned chunks */
     Workers *workers = nullptr;
 };
 
@@ -118,8 +126,11 @@ Staging *staging()
 {
     if (g_st) return g_st;
     Staging *st = new Staging();
+    const char *zc = getenv("CHAMAMD_XFER_ZC");
+    st->direct = !(zc && *zc == '0');
     for (int k = 0; k < XFER_CHUNKS; ++k) {
-        CHX_HIP(hipHostMalloc((void **)&st->host[k], XFER_CHUNK_BYTES, hipHostMallocDefault));
+        CHX_HIP(hipHostMalloc((void **)&st->host[k], XFER_CHUNK_BYTES, hipHostMallocMapped));
+        CHX_HIP(hipHostGetDevicePointer((void **)&st->hdev[k], st->host[k], 0));
         CHX_HIP(hipMalloc((void **)&st->dev[k], XFER_CHUNK_BYTES));
         CHX_HIP(hipEventCreateWithFlags(&st->ev[k], hipEventDisableTiming));
     }
@@ -183,9 +194,13 @@ void xfer_upload(CHAM_desc_t *A, Mirror *M, int part, hipStream_t s)
                 memcpy(h + (size_t)c * pc.rows, cm_at(A, pc.r0, jb + c), (size_t)pc.rows * sizeof(double));
         });
         const size_t bytes = (size_t)pc.rows * pc.cols * sizeof(double);
-        CHX_HIP(hipMemcpyAsync(st->dev[k], h, bytes, hipMemcpyHostToDevice, s));
-        k::panel_tiles((double *)st->dev[k], (double *)M->d, M->mt, M->ldt, M->tile_elems, A->mb, pc.n, pc.r0,
-                       pc.rows, pc.c0, pc.cols, 1, s);
+        double *src = (double *)st->hdev[k];
+        if (!st->direct) {
+            CHX_HIP(hipMemcpyAsync(st->dev[k], h, bytes, hipMemcpyHostToDevice, s));
+            src = (double *)st->dev[k];
+        }
+        k::panel_tiles(src, (double *)M->d, M->mt, M->ldt, M->tile_elems, A->mb, pc.n, pc.r0, pc.rows, pc.c0,
+                       pc.cols, 1, s);
         CHX_HIP(hipEventRecord(st->ev[k], s));
         st->used[k] = true;
     }
