@@ -370,6 +370,27 @@ def test_narrow_solves_vs_oracle(ch, n, nb, nrhs, uplo):
     assert oracle.check_solve(oracle.dplgsy(float(n), UL_, n, 51), B, oracle.dplrnt(n, nrhs, 5673))[0]
 
 
+@pytest.mark.parametrize("n,nb,nrhs", [(100, 64, 20), (129, 64, 33), (190, 64, 48), (200, 128, 64), (300, 128, 1),
+                                       (256, 128, 9)])
+@pytest.mark.parametrize("uplo", [L_, U_])
+def test_strip_solve_few_strips(ch, n, nb, nrhs, uplo):
+    """Two or three strips only (the chain's first and last strips are its
+    only ones; the predecessor products of 16- and 32-column chunks, the
+    48-column chunk without them, the narrow kernel with one 128-row strip
+    per tile and a partial last tile), dpotrf then dpotrs against the
+    oracle's dposv."""
+    say("few strips", n, nb, nrhs, uplo)
+    ch.set_param(ch.TILE_SIZE, nb)
+    A = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
+    B = oracle.dplrnt(n, nrhs, 5673, nb=nb)
+    assert ch.lib.CHAMELEON_dpotrf(uplo, n, _p(A), n) == 0
+    assert ch.lib.CHAMELEON_dpotrs(uplo, n, nrhs, _p(A), n, _p(B), n) == 0
+    Ar, Br, info = reference_solution(uplo, n, nrhs, nb)
+    assert info == 0
+    assert oracle.rel_frobenius(B, Br) <= TOL_SOLVE
+    assert oracle.check_solve(oracle.dplgsy(float(n), UL_, n, 51), B, oracle.dplrnt(n, nrhs, 5673))[0]
+
+
 @pytest.mark.parametrize("n,nb,nrhs", [(500, 512, 7), (1700, 256, 64), (1345, 192, 130), (2000, 320, 1),
                                        (1536, 128, 48), (3000, 256, 1), (2600, 512, 4), (3000, 384, 2)])
 @pytest.mark.parametrize("uplo", [L_, U_])
