@@ -306,6 +306,9 @@ void trsm_right_lower_trans_batched(double *P0, int ntiles, long long pstride, i
  * (n1 clamped to NT).  update_pairs returns how many pairs that is and sets
  * *ncol to the launch mode: the column count, or 0 for the whole triangle
  * n0 .. NT-1 (n1 >= NT and more than one column). */
+/* 1: the trailing-update kernels apply C -= product as FP64 atomic adds
+ * (same result, bit for bit); 0 (CHAMAMD_ATOMIC_UPDATE=0): load and store */
+int atomic_tile_updates();
 int update_pairs(int n0, int n1, int NT, int *ncol);
 /* pmap (NT x NT bytes, tile (m,n) at m + n*NT, or null): when given, only
  * the tiles marked 0 (FP64) are updated; the low-precision updates take

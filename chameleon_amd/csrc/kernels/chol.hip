@@ -75,7 +75,7 @@ __device__ __forceinline__ void lstore_slice(double *lds, const v2d (&r)[L], int
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update_kernel(
     double *A, int lmt, int ldt, int k, int kp, int n0, int NT, int ncol, int total_blocks, const int *info,
-    const unsigned char *pmap)
+    const unsigned char *pmap, int atomic_c)
 {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NTH = NW * 64;
@@ -191,7 +191,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void trailing_update
             for (int s = 0; s < 4; ++s) {
                 const int gi = row0 + wm + fm * 16 + (lane & 15);
                 const int gj = col0 + wn + fn * 16 + (lane >> 4) + 4 * s;
-                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= acc[fm][fn][s];
+                if (!diag || gi >= gj) c_sub(&C[gi + (size_t)gj * ldt], acc[fm][fn][s], atomic_c);
             }
 }
 
@@ -204,7 +204,18 @@ static void launch_update(double *A, int lmt, int ldt, int k, int kp, int n0, in
     if (total <= 0) return;
     constexpr int NTH = (BM / WM) * (BN / WN) * 64;
     hipLaunchKernelGGL((trailing_update_kernel<BM, BN, WM, WN>), dim3((unsigned)total), dim3(NTH), 0, s, A, lmt,
-                       ldt, k, kp, n0, NT, ncol, (int)total, info, pmap);
+                       ldt, k, kp, n0, NT, ncol, (int)total, info, pmap, atomic_tile_updates());
+}
+
+/* the tile updates subtract with FP64 atomic adds in the L2 (c_sub);
+ * CHAMAMD_ATOMIC_UPDATE=0: load, subtract and store */
+int atomic_tile_updates()
+{
+    static const int v = [] {
+        const char *e = getenv("CHAMAMD_ATOMIC_UPDATE");
+        return (e && *e == '0') ? 0 : 1;
+    }();
+    return v;
 }
 
 int update_pairs(int n0, int n1, int NT, int *ncol)

@@ -28,6 +28,17 @@ __device__ __forceinline__ v4f mfma_f32(float a, float b, v4f c)
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+/* C -= v for an element that this launch updates exactly once: either a
+ * load, subtract and store, or (atomic) one FP64 atomic add of -v done in
+ * the L2 without a return value -- the same rounding, bit for bit, and the
+ * wave does not wait for C's old value, so the tile's read-modify-write
+ * streams while the next blocks compute. */
+__device__ __forceinline__ void c_sub(double *p, double v, int atomic)
+{
+    if (atomic) atomicAdd(p, -v);
+    else *p -= v;
+}
+
 /* result row of slot s for lane l */
 __device__ __forceinline__ int mfma64_row(int l, int s) { return (l >> 4) + 4 * s; }
 __device__ __forceinline__ int mfma32_row(int l, int s) { return 4 * (l >> 4) + s; }

@@ -118,7 +118,7 @@ __device__ __forceinline__ void decode_pair(int pair, int n0, int ncol, int NT, 
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kernel(
     double *A, const float *P, const float *P2, int lmt, int ldt, int n0, int ncol, int total_blocks,
-    const int *info, const unsigned char *pmap)
+    const int *info, const unsigned char *pmap, int atomic_c)
 {
     constexpr int NW = (BM / WM) * (BN / WN), NTH = NW * 64, BK = 32;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void update_f32_kern
             for (int s = 0; s < 4; ++s) {
                 const int gi = row0 + wm + fm * 16 + (lane & 15);
                 const int gj = col0 + wn + fn * 16 + 4 * (lane >> 4) + s;
-                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= (double)acc[fm][fn][s];
+                if (!diag || gi >= gj) c_sub(&C[gi + (size_t)gj * ldt], (double)acc[fm][fn][s], atomic_c);
             }
 }
 
@@ -225,7 +225,7 @@ template <int SPLIT>
 __global__ __launch_bounds__(512, 4) void update_bf16_kernel(double *A, const unsigned short *P,
                                                              const unsigned short *P2, int NT, int lmt, int ldt,
                                                              int n0, int ncol, int total_blocks, const int *info,
-                                                             const unsigned char *pmap)
+                                                             const unsigned char *pmap, int atomic_c)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 64, LDK = BK;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(512, 4) void update_bf16_kernel(double *A, const un
             for (int s = 0; s < 4; ++s) {
                 const int gi = row0 + wm + fm * 16 + li;
                 const int gj = col0 + wn + fn * 16 + 4 * lk + s;
-                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= (double)acc[fm][fn][s];
+                if (!diag || gi >= gj) c_sub(&C[gi + (size_t)gj * ldt], (double)acc[fm][fn][s], atomic_c);
             }
 }
 
@@ -435,7 +435,8 @@ typedef int v4i32 __attribute__((ext_vector_type(4)));
  * same k assignment on both, so the sum over k is complete). */
 __global__ __launch_bounds__(512, 2) void update_fp8_kernel(double *A, const unsigned char *P,
                                                              const unsigned char *P2, int NT, int lmt, int ldt,
-                                                             int n0, int ncol, int total_blocks, const int *info)
+                                                             int n0, int ncol, int total_blocks, const int *info,
+                                                             int atomic_c)
 {
     constexpr int BM = 128, BN = 128, WM = 64, WN = 32, BK = 128;
     constexpr int FM = WM / 16, FN = WN / 16, NTH = 512;
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(512, 2) void update_fp8_kernel(double *A, const uns
             for (int s = 0; s < 4; ++s) {
                 const int gi = row0 + wm + fm * 16 + li;
                 const int gj = col0 + wn + fn * 16 + 4 * lk + s;
-                if (!diag || gi >= gj) C[gi + (size_t)gj * ldt] -= (double)acc[fm][fn][s];
+                if (!diag || gi >= gj) c_sub(&C[gi + (size_t)gj * ldt], (double)acc[fm][fn][s], atomic_c);
             }
 }
 
@@ -636,16 +637,17 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
     const int bpt = (ldt / 128) * (ldt / 128);
     const long long total = (long long)npairs * bpt;
     if (total <= 0) return;
+    const int ac = atomic_tile_updates();
     if (mode == 0)
         hipLaunchKernelGGL((update_f32_kernel<128, 128, 64, 32>), dim3((unsigned)total), dim3(512), 0, s, A,
-                           (const float *)P, (const float *)P2, lmt, ldt, n0, ncol, (int)total, info, pmap);
+                           (const float *)P, (const float *)P2, lmt, ldt, n0, ncol, (int)total, info, pmap, ac);
     else if (mode == 3)
         hipLaunchKernelGGL(update_fp8_kernel, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned char *)P,
-                           (const unsigned char *)P2, NT, lmt, ldt, n0, ncol, (int)total, info);
+                           (const unsigned char *)P2, NT, lmt, ldt, n0, ncol, (int)total, info, ac);
     else {
         auto *kern = mode == 1 ? update_bf16_kernel<1> : update_bf16_kernel<3>;
         hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(512), 0, s, A, (const unsigned short *)P,
-                           (const unsigned short *)P2, NT, lmt, ldt, n0, ncol, (int)total, info, pmap);
+                           (const unsigned short *)P2, NT, lmt, ldt, n0, ncol, (int)total, info, pmap, ac);
     }
 }
 
