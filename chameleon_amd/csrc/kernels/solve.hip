@@ -232,7 +232,7 @@ __device__ __forceinline__ void load_op4(const double *M, int ld, bool vec, int 
 /* polls of one flag wait before it gives up: about a second at ~0.3 us per
  * poll, far beyond any chain (a full direction at N = 32768 is ~3 ms) */
 #define STRIP_SPIN_LIMIT (1 << 22)
-template <int NC, bool PE>
+template <int NC, bool PE, bool FWD>
 __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
 {
     constexpr int NCOL = 16 * NC, NPC = SR / 16;
@@ -248,8 +248,11 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
     const int q = a.ldt / SR, nrp = a.nrp, ldt = a.ldt;
     const size_t ts = (size_t)ldt * ldt;
     const int wr = wave * 16;
-    const bool fwd = a.trans == 0;
-    const bool lvec = !fwd;       /* backward reads L(j, s)^T as row vectors */
+    /* the direction is a template parameter: with a run-time choice
+     * between the two operand loads the compiler put a wait for all
+     * outstanding loads before each 16-column piece of a factor block */
+    constexpr bool fwd = FWD;
+    constexpr bool lvec = !FWD;   /* backward reads L(j, s)^T as row vectors */
     /* task t = (strip in chain order) * nchunk + chunk: the chunks are
      * independent chains, interleaved so that the tasks a task waits for
      * always have smaller indices.  Tasks are taken from a ticket counter
@@ -285,8 +288,20 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
             for (int e = 0; e < 4; ++e) acc[c][e] = Bs[(c * 16 + li) + (size_t)(wr + lk + 4 * e) * nrp];
         /* D_s into LDS before any wait (read by the diagonal product) */
         const double *Dg = a.D + (size_t)tm * a.dts + (size_t)(s % q) * a.doff;
-#pragma unroll 4
-        for (int e = tid; e < SR * SR; e += 256) Ds[(e % SR) + (e / SR) * (SR + 1)] = Dg[(e % SR) + (size_t)(e / SR) * a.ldd];
+        {
+            /* all 16 loads in flight before the first LDS store */
+            double dv[SR * SR / 256];
+#pragma unroll
+            for (int h = 0; h < SR * SR / 256; ++h) {
+                const int e = tid + 256 * h;
+                dv[h] = Dg[(e % SR) + (size_t)(e / SR) * a.ldd];
+            }
+#pragma unroll
+            for (int h = 0; h < SR * SR / 256; ++h) {
+                const int e = tid + 256 * h;
+                Ds[(e % SR) + (e / SR) * (SR + 1)] = dv[h];
+            }
+        }
         const int pc0 = fwd ? 0 : wave, pc1 = fwd ? wave + 1 : NPC;   /* nonzero chunks of row group `wave` */
 
         int ready = fwd ? a.s0 - 1 : a.S;
@@ -465,21 +480,28 @@ __global__ __launch_bounds__(256, 2) void strip_solve_kernel(StripArgs a)
     }
 }
 
-template <int NC, bool PE>
-static void strip_launch(const StripArgs &a, hipStream_t s)
+template <int NC, bool PE, bool FWD>
+static void strip_launch_dir(const StripArgs &a, hipStream_t s)
 {
     static int per_cu = -1, ncu = 0;
     if (per_cu < 0) {
         int dev = 0;
         CHX_HIP(hipGetDevice(&dev));
         CHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, strip_solve_kernel<NC, PE>, 256, 0));
+        CHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, strip_solve_kernel<NC, PE, FWD>, 256, 0));
         if (per_cu < 1) per_cu = 1;
     }
     int grid = ncu * per_cu;
     if (grid > (a.S - a.s0) * a.nchunk) grid = (a.S - a.s0) * a.nchunk;
     zero_ints(reinterpret_cast<int *>(a.ticket), 1, s);
-    hipLaunchKernelGGL((strip_solve_kernel<NC, PE>), dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((strip_solve_kernel<NC, PE, FWD>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int NC, bool PE>
+static void strip_launch(const StripArgs &a, hipStream_t s)
+{
+    if (a.trans) strip_launch_dir<NC, PE, false>(a, s);
+    else strip_launch_dir<NC, PE, true>(a, s);
 }
 
 /* ------------------------------------------------------------------ */
