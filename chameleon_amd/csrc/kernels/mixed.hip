@@ -665,9 +665,13 @@ void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int 
  * one tile row).  Blocks below the diagonal tile are read as stored,
  * blocks above as the transpose of the stored lower tile (32-byte row
  * vectors), the diagonal tile by triangle.  The k loop runs in 16-deep
- * chunks with the next chunk's loads in flight. */
+ * chunks with the next chunk's loads in flight; the last chunk of a tile
+ * is peeled off the loop, and with FULL (every column chunk 64 wide) the
+ * column count is a constant, so the loop body has no branches: with
+ * run-time tests around each load and product the compiler split it into
+ * dozens of basic blocks. */
 #define RES_KS 8
-template <int RG>
+template <int RG, bool FULL>
 __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int NT, int ldt, const double *XT, int nrp,
                                                         double *part)
 {
@@ -684,7 +688,7 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
     const int r0 = rb * 16 * RG;
     const int m = r0 / ldt, i0 = r0 % ldt;
     const int j0 = jc * 64;
-    const int nf = min(4, (nrp - j0) / 16);
+    const int nf = FULL ? 4 : min(4, (nrp - j0) / 16);
     const size_t ts = (size_t)ldt * ldt;
     /* this wave's k range: [kb, ke) in units of 16 */
     const int nk16 = rows / 16;
@@ -724,26 +728,22 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
 #pragma unroll
                 for (int q = 0; q < 4; ++q) a[g][q] = T[16 * g + (size_t)(pb + 4 * lk + q) * ldt];
             load_x(nt * ldt + pb, x);
-            for (int p = pb; p < pe; p += 16) {
-                const bool more = p + 16 < pe;
-                if (more) {
+            for (int p = pb; p + 16 < pe; p += 16) {
 #pragma unroll
-                    for (int g = 0; g < RG; ++g)
+                for (int g = 0; g < RG; ++g)
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) an[g][q] = T[16 * g + (size_t)(p + 16 + 4 * lk + q) * ldt];
-                    load_x(nt * ldt + p + 16, xn);
-                }
+                    for (int q = 0; q < 4; ++q) an[g][q] = T[16 * g + (size_t)(p + 16 + 4 * lk + q) * ldt];
+                load_x(nt * ldt + p + 16, xn);
                 mma(a, x);
-                if (more) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < 4; ++q) {
 #pragma unroll
-                        for (int g = 0; g < RG; ++g) a[g][q] = an[g][q];
+                    for (int g = 0; g < RG; ++g) a[g][q] = an[g][q];
 #pragma unroll
-                        for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
-                    }
+                    for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
                 }
             }
+            mma(a, x);
         } else if (nt > m) {
             /* transpose of stored tile (nt, m): element (i0 + 16 g + li, p) at
              * T[p + 16 g ldt] */
@@ -756,28 +756,24 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
                 for (int q = 0; q < 4; ++q) a[g][q] = v[q];
             }
             load_x(nt * ldt + pb, x);
-            for (int p = pb; p < pe; p += 16) {
-                const bool more = p + 16 < pe;
-                if (more) {
+            for (int p = pb; p + 16 < pe; p += 16) {
 #pragma unroll
-                    for (int g = 0; g < RG; ++g) {
-                        const v4d v = *(const v4d *)(T + (size_t)16 * g * ldt + p + 16 + 4 * lk);
+                for (int g = 0; g < RG; ++g) {
+                    const v4d v = *(const v4d *)(T + (size_t)16 * g * ldt + p + 16 + 4 * lk);
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) an[g][q] = v[q];
-                    }
-                    load_x(nt * ldt + p + 16, xn);
+                    for (int q = 0; q < 4; ++q) an[g][q] = v[q];
                 }
+                load_x(nt * ldt + p + 16, xn);
                 mma(a, x);
-                if (more) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < 4; ++q) {
 #pragma unroll
-                        for (int g = 0; g < RG; ++g) a[g][q] = an[g][q];
+                    for (int g = 0; g < RG; ++g) a[g][q] = an[g][q];
 #pragma unroll
-                        for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
-                    }
+                    for (int f = 0; f < 4; ++f) x[q][f] = xn[q][f];
                 }
             }
+            mma(a, x);
         } else {
             /* diagonal tile, lower triangle stored */
             const double *T = A + ((size_t)m * NT + m) * ts;
@@ -827,11 +823,11 @@ void residual(const double *A, int NT, int ldt, const double *XT, const double *
 {
     /* two 16-row groups per wave when both lie in one tile row */
     const int rg = (ldt % 32 == 0) ? 2 : 1;
+    const bool full = nrp % 64 == 0;
     const long long blocks = (long long)((NT * ldt / (16 * rg) + 3) / 4) * ((nrp + 63) / 64) * RES_KS;
-    if (rg == 2)
-        hipLaunchKernelGGL(residual_kernel<2>, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
-    else
-        hipLaunchKernelGGL(residual_kernel<1>, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
+    auto kern = rg == 2 ? (full ? residual_kernel<2, true> : residual_kernel<2, false>)
+                        : (full ? residual_kernel<1, true> : residual_kernel<1, false>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, s, A, NT, ldt, XT, nrp, part);
     const size_t n = (size_t)NT * ldt * nrp;
     hipLaunchKernelGGL(residual_sum_kernel, dim3(2048), dim3(256), 0, s, BT, part, RT, n);
 }
