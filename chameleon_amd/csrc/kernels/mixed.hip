@@ -698,13 +698,21 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
     for (int g = 0; g < RG; ++g)
 #pragma unroll
         for (int f = 0; f < 4; ++f) acc[g][f] = zero4d();
-    /* x operands of chunk p0: X(p0 + 4 lk + q, j0 + 16 f + li) */
+    /* x operands of chunk p0: X(p0 + 4 lk + q, col(f, li)), where product f
+     * covers columns j0 + 16 f + li, or with FULL columns j0 + 4 li + f (a
+     * lane's four columns adjacent: one 32-byte load per k) */
     auto load_x = [&](int p0, double (&x)[4][4]) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double *xr = XT + (size_t)(p0 + 4 * lk + q) * nrp + j0 + li;
+            if constexpr (FULL) {
+                const v4d v = *(const v4d *)(XT + (size_t)(p0 + 4 * lk + q) * nrp + j0 + 4 * li);
 #pragma unroll
-            for (int f = 0; f < 4; ++f) x[q][f] = (f < nf) ? xr[16 * f] : 0.0;
+                for (int f = 0; f < 4; ++f) x[q][f] = v[f];
+            } else {
+                const double *xr = XT + (size_t)(p0 + 4 * lk + q) * nrp + j0 + li;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) x[q][f] = (f < nf) ? xr[16 * f] : 0.0;
+            }
         }
     };
     auto mma = [&](const double (&a)[RG][4], const double (&x)[4][4]) {
@@ -793,17 +801,25 @@ __global__ __launch_bounds__(256, 2) void residual_kernel(const double *A, int N
             }
         }
     }
-    /* slot s of acc[g][f]: row r0 + 16 g + lk + 4s, column j0 + 16 f + li */
+    /* slot s of acc[g][f]: row r0 + 16 g + lk + 4s, column col(f, li) */
     double *P = part + (size_t)ks * rows * nrp;
 #pragma unroll
-    for (int g = 0; g < RG; ++g)
+    for (int g = 0; g < RG; ++g) {
+        if constexpr (FULL) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f)
-            if (f < nf) {
+            for (int s = 0; s < 4; ++s)
+                *(v4d *)&P[(j0 + 4 * li) + (size_t)(r0 + 16 * g + lk + 4 * s) * nrp] =
+                    v4d{acc[g][0][s], acc[g][1][s], acc[g][2][s], acc[g][3][s]};
+        } else {
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    P[(j0 + 16 * f + li) + (size_t)(r0 + 16 * g + lk + 4 * s) * nrp] = acc[g][f][s];
-            }
+            for (int f = 0; f < 4; ++f)
+                if (f < nf) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        P[(j0 + 16 * f + li) + (size_t)(r0 + 16 * g + lk + 4 * s) * nrp] = acc[g][f][s];
+                }
+        }
+    }
 }
 
 __global__ void residual_sum_kernel(const double *BT, const double *part, double *RT, size_t n)
