@@ -892,6 +892,17 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
             for (int s = 0; s < 4; ++s) pf_st(xo[f][s], rR, vr, (64 * C + 16 * f + 4 * s) * ldt * 8);
         if (owner) {
             pf_publish(a, r * ND + C, tid);
+            /* at the first column block the diagonal block starts from A(r, r):
+             * its 16 values per lane requested together, ahead of the
+             * products (a load per value behind a run-time test put a
+             * wait after each one) */
+            double base0[4][4];
+            if (C == 0) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) base0[f][s] = pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8);
+            }
             /* rank-64 term X(r, C) X(r, C)^T of the diagonal block: the block
              * through LDS ([k][row]) as the shared operand */
 #pragma unroll
@@ -911,7 +922,7 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const int c = 16 * f + lk + 4 * s;
-                    const double base = C == 0 ? pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8) : Ld[rowo][c];
+                    const double base = C == 0 ? base0[f][s] : Ld[rowo][c];
                     Ld[rowo][c] = base - a2[f][s];
                 }
         }
@@ -921,14 +932,29 @@ __device__ __forceinline__ void panel_unit(const PanelArgs &a, int t, double *Ys
     /* diagonal block of owner r: A(r, r) - sum_p X(r, p) X(r, p)^T, lower part */
     double (*Lv)[OBP] = reinterpret_cast<double (*)[OBP]>(Ys);
     __syncthreads();
+    if (r == 0) {
+        /* owner 0 starts the chain from A(0, 0): all 16 loads in flight */
+        double v0[4][4];
 #pragma unroll
-    for (int f = 0; f < 4; ++f)
+        for (int f = 0; f < 4; ++f)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int c = 16 * f + lk + 4 * s;
-            const double v = r > 0 ? Ld[rowo][c] : pf_ld(rR, vr, (64 * r + 16 * f + 4 * s) * ldt * 8);
-            Ld[rowo][c] = (rowo >= c) ? v : 0.0;
-        }
+            for (int s = 0; s < 4; ++s) v0[f][s] = pf_ld(rR, vr, (16 * f + 4 * s) * ldt * 8);
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = 16 * f + lk + 4 * s;
+                Ld[rowo][c] = (rowo >= c) ? v0[f][s] : 0.0;
+            }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = 16 * f + lk + 4 * s;
+                if (rowo < c) Ld[rowo][c] = 0.0;
+            }
+    }
     __syncthreads();
     const int fail = pf_factor64(Ld, Lb, Lv, Rd, s_int, wave, lane);
     if (fail) {
