@@ -1037,11 +1037,22 @@ int solve_strips(const double *A, const double *D, long long dts, long long doff
  * strips of this tile (the earlier tiles were subtracted by fwd_update),
  * then X_b = D_b acc.  Each wave owns 16 rows of the strip; the MFMA
  * lane layout is the one of strip_solve_kernel (k index p0 + 4 lk + q).   */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void fwd_tile_kernel(const double *A, const double *D, double *BT, int k, int NT,
                                                        int ldt, int nrp, const int *info)
 {
     constexpr int NCOL = 16 * NC;
+    /* few registers and little LDS on purpose: the launch runs beside the
+     * bulk updates and has to fit a CU next to them */
     __shared__ double Xs[64 * NCOL];
     if (*info != 0) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1049,41 +1060,63 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(const double *A, const do
     const size_t ts = (size_t)ldt * ldt;
     const double *L = A + ((size_t)k * NT + k) * ts;
     double *B = BT + blockIdx.x * NCOL + (size_t)k * ldt * nrp;
+    /* buffer loads: a 32-bit offset per lane and a scalar one per load,
+     * instead of a 64-bit address per load */
+    const __amdgpu_buffer_rsrc_t rl = buf_rsrc(L), rb = buf_rsrc(B);
     for (int b = 0; b < ldt / 64; ++b) {
+        const int vl = ((64 * b + wr + li) + 4 * lk * ldt) * 8, vb = (li + 4 * lk * nrp) * 8;
         v4d acc[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[c][e] = B[(c * 16 + li) + (size_t)(64 * b + wr + lk + 4 * e) * nrp];
-        for (int j = 0; j < b; ++j) {
+        /* D_b operands of this wave's rows, requested before the products
+         * (the blocks above the diagonal are zero and not multiplied) */
+        const double *Db = D + (size_t)b * 4096;
+        double dv[4][4];
 #pragma unroll
-            for (int pc = 0; pc < 4; ++pc) {
-                double l[4];
-                load_op4(L, ldt, false, 64 * b + wr, 64 * j + pc * 16, li, lk, l);
+        for (int pc = 0; pc < 4; ++pc) load_op4(Db, 64, false, wr, pc * 16, li, lk, dv[pc]);
+        /* acc -= L(b, j) X_j over the earlier strips: per strip j all its
+         * operands (the factor block and the solution rows X_j, written by
+         * this workgroup before the barrier that ended strip j) requested
+         * together, then the 16 products (operand loads inside the product
+         * loop made every product wait for its own load) */
+        for (int j = 0; j < b; ++j) {
+            double l[4][4], xo[4][4][NC];
+#pragma unroll
+            for (int pc = 0; pc < 4; ++pc)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const size_t row = (size_t)(64 * j + pc * 16 + 4 * lk + q) * nrp;
+                    /* L(64 b + wr + li, 64 j + pc 16 + 4 lk + q) and
+                     * X(64 j + pc 16 + 4 lk + q, c 16 + li) */
+                    const int kk = 64 * j + pc * 16 + q;
+                    l[pc][q] = buf_ld(rl, vl, kk * ldt * 8);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) acc[c] = mfma_f64(-l[q], B[(c * 16 + li) + row], acc[c]);
+                    for (int c = 0; c < NC; ++c) xo[pc][q][c] = buf_ld(rb, vb + c * 128, kk * nrp * 8);
                 }
-            }
+#pragma unroll
+            for (int pc = 0; pc < 4; ++pc)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) acc[c] = mfma_f64(-l[pc][q], xo[pc][q][c], acc[c]);
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int e = 0; e < 4; ++e) Xs[(wr + lk + 4 * e) * NCOL + c * 16 + li] = acc[c][e];
         __syncthreads();
-        const double *Db = D + (size_t)b * 4096;
         v4d x[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) x[c] = zero4d();
-        for (int pc = 0; pc <= wave; ++pc) {
+#pragma unroll
+        for (int pc = 0; pc < 4; ++pc) {
+            if (pc > wave) continue;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int pp = pc * 16 + 4 * lk + q;
-                const double dv = Db[(wr + li) + pp * 64];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv, Xs[pp * NCOL + c * 16 + li], x[c]);
+                for (int c = 0; c < NC; ++c) x[c] = mfma_f64(dv[pc][q], Xs[pp * NCOL + c * 16 + li], x[c]);
             }
         }
 #pragma unroll
