@@ -358,6 +358,7 @@ void panel_demote_fp8(const double *A, int NT, int ldt, int k, int m0, int m1, v
  * lower tiles of A scaled by sc_i sc_j; scale_bt: rows of transposed
  * right-hand sides (element (c, r) at c + r*nrp) times sc[r] */
 void diag_scaling(double *A, int NT, int ldt, double *sc, hipStream_t s);
+void diag_rsqrt(const double *A, int NT, int ldt, double *sc, hipStream_t s);   /* sc only */
 void scale_bt(double *BT, int nrp, int rows, const double *sc, hipStream_t s);
 void trailing_update_lp(double *A, const void *P, const void *P2, int mode, int lmt, int ldt, int n0, int n1, int NT,
                         const int *info, hipStream_t s, const unsigned char *pmap = nullptr);
@@ -370,12 +371,14 @@ void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStre
  * (max-combined into out: clear it first) */
 void colmax2(const double *X, const double *R, int nrp, int ncols, int nrows, double *out, hipStream_t s);
 void axpy(double *Y, const double *X, size_t n, hipStream_t s);
-/* lower tiles of S copied to W and the symmetric inf-norm's row sums into
- * rowsum[N]; returns -1 (nothing launched) unless ldt is a multiple of 128
- * up to COPY_NORM_MAX_LDT and the tiling of N by mb fills NT tile rows */
+/* lower tiles of S copied to W (with sc: scaled by sc_i sc_j as
+ * diag_scaling does, in the same pass) and the symmetric inf-norm's row
+ * sums of S into rowsum[N]; returns -1 (nothing launched) unless ldt is a
+ * multiple of 128 up to COPY_NORM_MAX_LDT and the tiling of N by mb fills
+ * NT tile rows */
 #define COPY_NORM_MAX_LDT 2048
 int copy_lower_norm(const double *S, double *W, int NT, int ldt, int mb, int N, double *part, double *rowsum,
-                    hipStream_t s);
+                    hipStream_t s, const double *sc = nullptr);
 size_t copy_lower_norm_scratch(int NT, int ldt);     /* doubles of `part` */
 /* latms.hip: square tile mirrors of NT x NT tiles, ldt x ldt, mb valid rows per tile */
 void gauss(double *G, int NT, int ldt, int mb, int N, unsigned long long seed, hipStream_t s);

@@ -575,9 +575,14 @@ __global__ void scale_bt_kernel(double *BT, int nrp, int rows, const double *sc)
         BT[e] *= sc[e / nrp];
 }
 
-void diag_scaling(double *A, int NT, int ldt, double *sc, hipStream_t s)
+void diag_rsqrt(const double *A, int NT, int ldt, double *sc, hipStream_t s)
 {
     hipLaunchKernelGGL(diag_rsqrt_kernel, dim3((NT * ldt + 255) / 256), dim3(256), 0, s, A, NT, ldt, sc);
+}
+
+void diag_scaling(double *A, int NT, int ldt, double *sc, hipStream_t s)
+{
+    diag_rsqrt(A, NT, ldt, sc, s);
     hipLaunchKernelGGL(scale_lower_kernel, dim3(std::min(64, (ldt * ldt + 255) / 256), NT * (NT + 1) / 2), dim3(256),
                        0, s, A, NT, ldt, sc);
 }
@@ -912,7 +917,7 @@ void axpy(double *Y, const double *X, size_t n, hipStream_t s)
  * through a wave reduction into an LDS row of the wave's own (no atomics:
  * the result does not depend on scheduling).  ldt a multiple of 128. */
 __global__ __launch_bounds__(256) void copy_lower_norm_kernel(const double *S, double *W, int NT, int ldt, int mb,
-                                                              int N, double *part)
+                                                              int N, double *part, const double *sc)
 {
     extern __shared__ double cs[];      /* [4][ldt] column sums, one row per wave */
     const int m = blockIdx.x % NT, n = blockIdx.x / NT;
@@ -929,6 +934,9 @@ __global__ __launch_bounds__(256) void copy_lower_norm_kernel(const double *S, d
     for (int r0 = wave * 128; r0 < ldt; r0 += 512) {
         const int i = r0 + 2 * lane;
         double rs0 = 0.0, rs1 = 0.0;
+        /* with sc: the copy is S A S (element (i, j) times s_i s_j, the
+         * product of the two scales first), the sums stay those of A */
+        const double sr0 = sc ? sc[m * ldt + i] : 1.0, sr1 = sc ? sc[m * ldt + i + 1] : 1.0;
         for (int j0 = 0; j0 < ldt; j0 += 8) {
             double2 v[8];
 #pragma unroll
@@ -936,7 +944,12 @@ __global__ __launch_bounds__(256) void copy_lower_norm_kernel(const double *S, d
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int j = j0 + u;
-                *reinterpret_cast<double2 *>(dst + (size_t)j * ldt + i) = v[u];
+                if (sc) {
+                    const double sj = sc[n * ldt + j];
+                    *reinterpret_cast<double2 *>(dst + (size_t)j * ldt + i) = double2{v[u].x * (sr0 * sj), v[u].y * (sr1 * sj)};
+                } else {
+                    *reinterpret_cast<double2 *>(dst + (size_t)j * ldt + i) = v[u];
+                }
                 const bool cj = j < vn;
                 const double a0 = (cj && i < vm && (!diag || i >= j)) ? fabs(v[u].x) : 0.0;
                 const double a1 = (cj && i + 1 < vm && (!diag || i + 1 >= j)) ? fabs(v[u].y) : 0.0;
@@ -972,13 +985,13 @@ __global__ void norm_rows_kernel(const double *part, int NT, int ldt, int mb, in
 size_t copy_lower_norm_scratch(int NT, int ldt) { return (size_t)2 * NT * NT * ldt; }
 
 int copy_lower_norm(const double *S, double *W, int NT, int ldt, int mb, int N, double *part, double *rowsum,
-                    hipStream_t s)
+                    hipStream_t s, const double *sc)
 {
     if (NT <= 0 || ldt % 128 != 0 || ldt > COPY_NORM_MAX_LDT || mb <= 0 || mb > ldt || N <= (NT - 1) * mb ||
         N > NT * mb)
         return -1;
     hipLaunchKernelGGL(copy_lower_norm_kernel, dim3((unsigned)(NT * NT)), dim3(256), (size_t)4 * ldt * sizeof(double),
-                       s, S, W, NT, ldt, mb, N, part);
+                       s, S, W, NT, ldt, mb, N, part, sc);
     hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)std::min(2048, (N + 255) / 256)), dim3(256), 0, s, part, NT,
                        ldt, mb, N, rowsum);
     return 0;

@@ -226,8 +226,13 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         double *tss = sc + rows;      /* tile norms of the per-tile precision choice */
         /* the factorisation reads and writes the lower tiles only: copy
          * those and form ||A||_inf in the same pass (row sums into RT) */
+        /* the working copy becomes S A S, S = diag(A)^-1/2 (a solve with
+         * it is X = S (S A S)^-1 S B): the scales from A's diagonal first,
+         * then applied by the copy itself */
+        const bool scaled = c->mixed_scaling != 0;
+        if (scaled) k::diag_rsqrt((const double *)MA->d, NT, ldt, sc, s);
         double anrm = 0.0;
-        if (k::copy_lower_norm((const double *)MA->d, Aw, NT, ldt, A->mb, N, part, RT, s) == 0) {
+        if (k::copy_lower_norm((const double *)MA->d, Aw, NT, ldt, A->mb, N, part, RT, s, scaled ? sc : nullptr) == 0) {
             std::vector<double> h(N);
             CHX_HIP(hipMemcpyAsync(h.data(), RT, N * sizeof(double), hipMemcpyDeviceToHost, s));
             CHX_HIP(hipStreamSynchronize(s));
@@ -235,12 +240,9 @@ int mixed_posv(cham_uplo_t uplo, CHAM_desc_t *A, CHAM_desc_t *B, CHAM_desc_t *X,
         } else {
             anrm = norm_inf_sym(A);
             CHX_HIP(hipMemcpyAsync(Aw, MA->d, abytes, hipMemcpyDeviceToDevice, s));
+            if (scaled) k::diag_scaling(Aw, NT, ldt, sc, s);
         }
         const double cte = anrm * eps * std::sqrt((double)N);
-        /* the working copy becomes S A S, S = diag(A)^-1/2; a solve with
-         * it is X = S (S A S)^-1 S B */
-        const bool scaled = c->mixed_scaling != 0;
-        if (scaled) k::diag_scaling(Aw, NT, ldt, sc, s);
         if (adaptive) tile_precision_map(Aw, NT, ldt, tss, dmap, s);
 
         k::rhs_transpose((double *)MB->d, BT, NT, ldt, MB->wdt, MB->nt, 1, s);

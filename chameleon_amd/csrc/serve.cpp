@@ -437,12 +437,14 @@ void launch_large_factor(Server *s, Slot &sl, Job *job)
         }
     }
     if (keep) {
-        /* working copy of the lower tiles and ||A||_inf in one pass */
+        /* working copy of the lower tiles (mixed: scaled by diag(A)^-1/2 on
+         * both sides in the same pass) and ||A||_inf in one pass */
         double *rowsum = (double *)(dv + o_rows);
-        if (k::copy_lower_norm(optr(s, f), W, NT, ldt, f->nb, f->n, (double *)(dv + o_part), rowsum, st) != 0)
+        if (mixed) k::diag_rsqrt(optr(s, f), NT, ldt, scptr(s, f), st);
+        if (k::copy_lower_norm(optr(s, f), W, NT, ldt, f->nb, f->n, (double *)(dv + o_part), rowsum, st,
+                               mixed ? scptr(s, f) : nullptr) != 0)
             fatal_hip(hipErrorInvalidValue, "copy_lower_norm", __FILE__, __LINE__);
         CHX_HIP(hipMemcpyAsync(h + o_hrows, rowsum, f->n * sizeof(double), hipMemcpyDeviceToHost, st));
-        if (mixed) k::diag_scaling(W, NT, ldt, scptr(s, f), st);
     }
     k::zero_ints(sl.d_info, 1, st);
     PotrfPlan p;
