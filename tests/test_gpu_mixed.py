@@ -69,7 +69,7 @@ def mixed(ch, uplo, A, B, prec, nb):
 @pytest.mark.parametrize("prec", [FP32, BF16X3, BF16, FP8])
 @pytest.mark.parametrize("uplo", [L_, U_])
 @pytest.mark.parametrize("n,nb,nrhs", [(1024, 256, 1), (1536, 256, 16), (2048, 512, 64), (1000, 128, 3),
-                                       (1150, 128, 7), (2500, 256, 32)])
+                                       (1150, 128, 7), (2500, 256, 32), (1536, 128, 128)])
 def test_mixed_well_conditioned(ch, prec, uplo, n, nb, nrhs):
     say("mixed bumped", prec, uplo, n, nb, nrhs)
     A = oracle.dplgsy(float(n), uplo, n, 51, nb=nb)
