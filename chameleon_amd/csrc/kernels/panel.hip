@@ -221,6 +221,78 @@ struct PotrfJobs {
     int iinfo, batch;
 };
 
+/* Phase (a) of potrf_jobs_kernel (below) for a full 64-column block C: the
+ * same products in the same order as its general loop, without the
+ * run-time tests of that loop (block width, last slice, rows past the
+ * matrix) around the slice loads; the last slice is peeled off.  A wave
+ * whose rows lie past the matrix reads row li instead and discards it. */
+template <int POTRF_WAVES, bool LP>
+__device__ __forceinline__ void potrf_update_full(double *T, int ldt, int C, int r0, bool active,
+                                                  double (*Ys)[16 * OB], int tid, int li, int lk)
+{
+    constexpr int POTRF_THREADS = POTRF_WAVES * 64, YH = 512 / POTRF_THREADS, YS = POTRF_THREADS / 32;
+    v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+    v4f accf[4] = {zero4f(), zero4f(), zero4f(), zero4f()};
+    const double *px = T + (active ? r0 + li : li) + (size_t)lk * ldt;
+    const int yq = tid >> 5, yr = (tid & 31) * 2;
+    const double *py = T + (C + yr) + (size_t)yq * ldt;
+    auto products = [&](int cur, const double (&x)[4]) {
+        if (!active) return;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                if constexpr (LP)
+                    accf[f] = mfma_f32((float)Ys[cur][(4 * q + lk) * OB + 16 * f + li], (float)x[q], accf[f]);
+                else
+                    acc[f] = mfma_f64(Ys[cur][(4 * q + lk) * OB + 16 * f + li], x[q], acc[f]);
+            }
+    };
+    v2d yreg[YH];
+    double x[4];
+#pragma unroll
+    for (int h = 0; h < YH; ++h) yreg[h] = *reinterpret_cast<const v2d *>(py + (size_t)(h * YS) * ldt);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double v = px[(size_t)(4 * q) * ldt];
+        x[q] = active ? v : 0.0;
+    }
+#pragma unroll
+    for (int h = 0; h < YH; ++h) *reinterpret_cast<v2d *>(&Ys[0][(yq + h * YS) * OB + yr]) = yreg[h];
+    __syncthreads();
+    int p = 0;
+    for (; p + 16 < C; p += 16) {
+        const int cur = (p >> 4) & 1;
+        double xn[4];
+#pragma unroll
+        for (int h = 0; h < YH; ++h)
+            yreg[h] = *reinterpret_cast<const v2d *>(py + (size_t)(p + 16 + h * YS) * ldt);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double v = px[(size_t)(p + 16 + 4 * q) * ldt];
+            xn[q] = active ? v : 0.0;
+        }
+        products(cur, x);
+#pragma unroll
+        for (int h = 0; h < YH; ++h) *reinterpret_cast<v2d *>(&Ys[cur ^ 1][(yq + h * YS) * OB + yr]) = yreg[h];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = xn[q];
+        __syncthreads();
+    }
+    products((p >> 4) & 1, x);
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int r = r0 + li;
+                const int c = C + 16 * f + (LP ? 4 * lk + s : lk + 4 * s);
+                if (r >= c) T[r + (size_t)c * ldt] -= LP ? (double)accf[f][s] : acc[f][s];
+            }
+    }
+}
+
 template <int POTRF_WAVES, bool LP = false>
 __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_kernel(PotrfJobs J)
 {
@@ -266,6 +338,10 @@ __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_
                 const int rb = rb0 + wave;
                 const bool active = rb < nrb;
                 const int r0 = C + rb * 16;
+                if (W == OB) {
+                    potrf_update_full<POTRF_WAVES, LP>(T, ldt, C, r0, active, Ys, tid, li, lk);
+                    continue;
+                }
                 v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
                 /* LP: the same products with FP32 operands and accumulation
                  * (v_mfma_f32_16x16x4_f32, result slot s at column 4 lk + s) */
