@@ -495,6 +495,23 @@ __global__ __launch_bounds__(POTRF_WAVES * 64, 8 / POTRF_WAVES) void potrf_jobs_
             for (int rb = wave; rb < nrb; rb += POTRF_WAVES) {
                 const int r0 = C + W + rb * 16;
                 double pv[16];
+                if (W == OB) {
+                    /* full block: the 16 loads with no run-time test between
+                     * them (in flight together), the same products */
+#pragma unroll
+                    for (int kq = 0; kq < 16; ++kq) pv[kq] = T[(r0 + li) + (size_t)(C + 4 * kq + lk) * ldt];
+                    v4d acc[4] = {zero4d(), zero4d(), zero4d(), zero4d()};
+#pragma unroll
+                    for (int f = 0; f < 4; ++f)
+#pragma unroll
+                        for (int kq = 0; kq < 4 * (f + 1); ++kq)
+                            acc[f] = mfma_f64(Lv[16 * f + li][4 * kq + lk], pv[kq], acc[f]);
+#pragma unroll
+                    for (int f = 0; f < 4; ++f)
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) T[(r0 + li) + (size_t)(C + 16 * f + lk + 4 * s) * ldt] = acc[f][s];
+                    continue;
+                }
 #pragma unroll
                 for (int kq = 0; kq < 16; ++kq)
                     pv[kq] = (4 * kq < W) ? T[(r0 + li) + (size_t)(C + 4 * kq + lk) * ldt] : 0.0;
