@@ -282,13 +282,24 @@ __device__ __forceinline__ void potrf_update_full(double *T, int ldt, int C, int
     products((p >> 4) & 1, x);
     __syncthreads();
     if (active) {
+        /* the 16 old values first, all in flight (a load-subtract-store per
+         * element waited for each load in turn); the block lies inside the
+         * matrix, so the upper-triangle ones are read and left as they are */
+        double old[4][4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = C + 16 * f + (LP ? 4 * lk + s : lk + 4 * s);
+                old[f][s] = T[(r0 + li) + (size_t)c * ldt];
+            }
 #pragma unroll
         for (int f = 0; f < 4; ++f)
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int r = r0 + li;
                 const int c = C + 16 * f + (LP ? 4 * lk + s : lk + 4 * s);
-                if (r >= c) T[r + (size_t)c * ldt] -= LP ? (double)accf[f][s] : acc[f][s];
+                if (r >= c) T[r + (size_t)c * ldt] = old[f][s] - (LP ? (double)accf[f][s] : acc[f][s]);
             }
     }
 }
