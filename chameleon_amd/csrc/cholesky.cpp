@@ -481,6 +481,7 @@ void potrf_solve_run(PotrfPlan p, double *BT, int nrp, int nrhs, double rate)
     f.NT = NT; f.ldt = ldt; f.nrp = nrp; f.nrhs = nrhs; f.nb = p.nb;
     f.next = 0;
     f.last = c->fwd_tail >= 0 ? NT - 1 - c->fwd_tail : fwd_overlap_last(NT, p.nb, rate);
+    if (!k::fwd_tile_fits(ldt, nrp)) f.last = -1;    /* all rows by the strip solve below */
     f.d_info = p.d_info;
     p.after_panels = fwd_after_panels;
     p.after_arg = &f;

@@ -418,6 +418,12 @@ void solve_coop(const double *A, const double *TN, const double *TT, double *BT,
 void rhs_transpose(double *Bm, double *BT, int NT, int ldt, int wdt, int bnt, int to_bt, hipStream_t s,
                    const int *info = nullptr);
 /* forward substitution of one tile row right after its panel (dposv) */
+/* the forward tile solve addresses a tile row of the factor and of the
+ * right-hand sides with 32-bit buffer offsets */
+inline bool fwd_tile_fits(int ldt, int nrp)
+{
+    return (size_t)ldt * ldt * 8 < ((size_t)1 << 31) && (size_t)ldt * nrp * 8 < ((size_t)1 << 31);
+}
 void fwd_tile(const double *A, const double *D, double *BT, int k, int NT, int ldt, int nrp, const int *info,
               hipStream_t s);
 void fwd_update(const double *A, double *BT, int k, int NT, int ldt, int nrp, const int *info, hipStream_t s);

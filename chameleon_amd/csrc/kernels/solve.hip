@@ -1172,6 +1172,8 @@ void fwd_tile(const double *A, const double *D, double *BT, int k, int NT, int l
               hipStream_t s)
 {
     /* 16-column chains: the most workgroups on this short latency chain */
+    if (!fwd_tile_fits(ldt, nrp))   /* callers keep such shapes off this path */
+        ::chx::fatal_hip(hipErrorInvalidValue, "fwd_tile: offsets past 32 bits", __FILE__, __LINE__);
     hipLaunchKernelGGL(fwd_tile_kernel<1>, dim3(nrp / 16), dim3(256), 0, s, A, D, BT, k, NT, ldt, nrp, info);
 }
 
