@@ -872,22 +872,36 @@ void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStre
 }
 
 /* the column maxima of X (out[j]) and of R (out[nrp + j]) in one launch */
-__global__ void colmax2_kernel(const double *X, const double *R, int nrp, int nrows, double *out)
+/* lane c of wave g: column 64 blockIdx.y + c, rows g, g + 4, ... of this
+ * block's share: a wave reads 64 adjacent columns of a row (coalesced; one
+ * column per workgroup had each lane touch its own 512-byte line), the four
+ * waves meet in LDS, one atomic max per column and array per workgroup */
+__global__ void colmax2_kernel(const double *X, const double *R, int nrp, int ncols, int nrows, double *out)
 {
-    const int j = blockIdx.y;
-    const double *V = blockIdx.z ? R : X;
-    double mx = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nrows; p += gridDim.x * blockDim.x)
-        mx = nan_max(mx, fabs(V[j + (size_t)p * nrp]));
-    for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off));
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(reinterpret_cast<unsigned long long *>(&out[blockIdx.z * nrp + j]),
-                  (unsigned long long)__double_as_longlong(mx));
+    __shared__ double sx[4][64], sr[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int j = blockIdx.y * 64 + c;
+    double mx = 0.0, mr = 0.0;
+    if (j < ncols) {
+        for (int p = blockIdx.x * 4 + g; p < nrows; p += gridDim.x * 4) {
+            mx = nan_max(mx, fabs(X[j + (size_t)p * nrp]));
+            mr = nan_max(mr, fabs(R[j + (size_t)p * nrp]));
+        }
+    }
+    sx[g][c] = mx;
+    sr[g][c] = mr;
+    __syncthreads();
+    if (g == 0 && j < ncols) {
+        mx = nan_max(nan_max(sx[0][c], sx[1][c]), nan_max(sx[2][c], sx[3][c]));
+        mr = nan_max(nan_max(sr[0][c], sr[1][c]), nan_max(sr[2][c], sr[3][c]));
+        atomicMax(reinterpret_cast<unsigned long long *>(&out[j]), (unsigned long long)__double_as_longlong(mx));
+        atomicMax(reinterpret_cast<unsigned long long *>(&out[nrp + j]), (unsigned long long)__double_as_longlong(mr));
+    }
 }
 
 void colmax2(const double *X, const double *R, int nrp, int ncols, int nrows, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(colmax2_kernel, dim3(64, ncols, 2), dim3(256), 0, s, X, R, nrp, nrows, out);
+    hipLaunchKernelGGL(colmax2_kernel, dim3(128, (ncols + 63) / 64), dim3(256), 0, s, X, R, nrp, ncols, nrows, out);
 }
 
 __global__ void axpy_kernel(double *Y, const double *X, size_t n)
