@@ -366,7 +366,6 @@ void tile_sumsq_lower(const double *A, int NT, int ldt, double *ss, hipStream_t 
 void residual(const double *A, int NT, int ldt, const double *XT, const double *BT, double *RT, int nrp, double *part,
               hipStream_t s);
 size_t residual_scratch(int NT, int ldt, int nrp);   /* doubles of `part` */
-void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStream_t s);
 /* column maxima of X into out[0..ncols) and of R into out[nrp..nrp+ncols)
  * (max-combined into out: clear it first) */
 void colmax2(const double *X, const double *R, int nrp, int ncols, int nrows, double *out, hipStream_t s);

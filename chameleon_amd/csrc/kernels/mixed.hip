@@ -853,29 +853,14 @@ void residual(const double *A, int NT, int ldt, const double *XT, const double *
     hipLaunchKernelGGL(residual_sum_kernel, dim3(2048), dim3(256), 0, s, BT, part, RT, n);
 }
 
-/* per-column max |V(j, p)| over p < nrows (V transposed: j + p*nrp); out
- * zeroed by the caller */
-__global__ void colmax_kernel(const double *V, int nrp, int nrows, double *out)
-{
-    const int j = blockIdx.y;
-    double mx = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nrows; p += gridDim.x * blockDim.x)
-        mx = nan_max(mx, fabs(V[j + (size_t)p * nrp]));
-    for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off));
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(reinterpret_cast<unsigned long long *>(&out[j]), (unsigned long long)__double_as_longlong(mx));
-}
-
-void colmax(const double *V, int nrp, int ncols, int nrows, double *out, hipStream_t s)
-{
-    hipLaunchKernelGGL(colmax_kernel, dim3(64, ncols), dim3(256), 0, s, V, nrp, nrows, out);
-}
-
-/* the column maxima of X (out[j]) and of R (out[nrp + j]) in one launch */
-/* lane c of wave g: column 64 blockIdx.y + c, rows g, g + 4, ... of this
- * block's share: a wave reads 64 adjacent columns of a row (coalesced; one
- * column per workgroup had each lane touch its own 512-byte line), the four
- * waves meet in LDS, one atomic max per column and array per workgroup */
+/* the column maxima max_p |V(j, p)| (V transposed: element (j, p) at
+ * j + p*nrp) of X into out[j] and of R into out[nrp + j], in one launch;
+ * out cleared by the caller.  Lane c of wave g: column 64 blockIdx.y + c,
+ * rows g, g + 4, ... of this block's share: a wave reads 64 adjacent
+ * columns of a row (coalesced; one column per workgroup had each lane
+ * touch its own 512-byte line), the four waves meet in LDS, one atomic max
+ * per column and array per workgroup (on the bit pattern: order-free and
+ * NaN-propagating for these non-negative values) */
 __global__ void colmax2_kernel(const double *X, const double *R, int nrp, int ncols, int nrows, double *out)
 {
     __shared__ double sx[4][64], sr[4][64];
